@@ -1,0 +1,204 @@
+/*
+ * mplbm — C ABI of the MI355X (gfx950) D3Q19 Shan-Chen multiphase
+ * lattice-Boltzmann step.  libmplbm.so exports exactly the functions below;
+ * no C++ or torch types cross this boundary.
+ *
+ * The reference drives its GPU path through three host functions
+ * (include/iteration_host.h:30-60, src/iteration_host.cu:16-81), all of
+ * them with compile-time domain sizes and device pointers owned by the
+ * caller.  This ABI replaces them with a context that owns device memory,
+ * takes runtime sizes, reports errors through status codes (the reference
+ * calls exit(), include/iteration_host.h:6-15) and runs on its own HIP
+ * stream.  iteration_host_compat.h keeps the three reference names on top of
+ * this ABI.
+ *
+ * Field arrays crossing the ABI use the reference layout: [x][y][z], z
+ * fastest (include/lbm_model.cuh:18-22); populations [x][y][z][19] in the
+ * reference's velocity order (include/lbm_model.cuh:49-54).
+ *
+ * Threading: a context belongs to one host thread at a time.  mplbm_step is
+ * asynchronous on the context stream; every call that returns host data
+ * synchronises that stream first.
+ */
+#ifndef MPLBM_H_
+#define MPLBM_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum mplbm_status {
+  MPLBM_OK = 0,
+  MPLBM_ERR_ARG = 1,         /* invalid argument or configuration */
+  MPLBM_ERR_HIP = 2,         /* a HIP runtime call failed */
+  MPLBM_ERR_STATE = 3,       /* call out of order (e.g. step before init) */
+  MPLBM_ERR_NOMEM = 4,       /* device allocation failed */
+  MPLBM_ERR_UNSUPPORTED = 5  /* option not available for this storage type */
+} mplbm_status;
+
+/* Population storage.  FP32 and FP64 evaluate the reference arithmetic
+ * (FP32: the reference's own float storage with its double-promoted
+ * sub-expressions, include/iteration_device.cuh:7-206).  FP16S/BF16S store
+ * the shifted, scaled deviation (f_i/w_i - rho_ref)*scale in 16 bits and do
+ * all arithmetic in FP32 registers. */
+typedef enum mplbm_storage {
+  MPLBM_FP32 = 0,
+  MPLBM_FP64 = 1,
+  MPLBM_FP16S = 2,
+  MPLBM_BF16S = 3
+} mplbm_storage;
+
+/* Step kernel.  FUSED: one marching stream-collide launch per step with the
+ * pseudopotential halo in LDS.  TWOPASS: a density/psi launch followed by a
+ * force/collision launch (the schedule the reference's do_collision needs
+ * to be race-free). */
+typedef enum mplbm_kernel {
+  MPLBM_KERNEL_AUTO = 0,
+  MPLBM_KERNEL_FUSED = 1,
+  MPLBM_KERNEL_TWOPASS = 2
+} mplbm_kernel;
+
+/* Outer-face types (index order xmin, xmax, ymin, ymax, zmin, zmax).  The
+ * reference's face flags (include/global_var.cuh:39-63) are fixed to
+ * periodic; PRESSURE prescribes face_rho[f] through the anti-bounce-back
+ * rule of include/iteration_device.cuh:213-243. */
+typedef enum mplbm_face { MPLBM_FACE_PERIODIC = 0, MPLBM_FACE_PRESSURE = 1 } mplbm_face;
+
+/* All fields of transfer_parameter (src/iteration_host.cu:16-41) plus the
+ * domain size, storage, faces, body force and kernel tiling.  Fill with
+ * mplbm_config_default() and override. */
+typedef struct mplbm_config {
+  int64_t lx, ly, lz;          /* domain size (compile-time 400^3 in the reference) */
+  int32_t storage;             /* mplbm_storage */
+  int32_t kernel;              /* mplbm_kernel */
+  double tau;                  /* BGK relaxation time */
+  double G;                    /* Shan-Chen interaction strength */
+  int32_t carn_star;           /* 1: Carnahan-Starling EOS, 0: Shan-Chen */
+  int32_t reserved0;
+  double T_Tc;                 /* CS EOS T/Tc */
+  double rhol_spinodal;        /* liquid threshold (injection types 1/2, stats) */
+  double rhog_spinodal;        /* vapour threshold (stats) */
+  int64_t rho_inject_period;   /* injection fires when step % period == 0 (0: never) */
+  double rho_increment;        /* injection increment */
+  double rhos;                 /* solid density (wetting) */
+  double inject_position;      /* injection applies where y < inject_position*ly */
+  int32_t face_type[6];        /* mplbm_face per face */
+  double face_rho[6];          /* prescribed density of PRESSURE faces */
+  double body_force[3];        /* acceleration g; the force density rho*g is added to
+                                  the Shan-Chen force (0 reproduces the reference) */
+  double rho_ref;              /* FP16S/BF16S shift; 0 = choose from the initial field */
+  double dev_scale;            /* FP16S/BF16S scale; 0 = choose automatically */
+  int32_t tile_y, tile_z;      /* FUSED: threads per tile in y and z (0 = tuned default) */
+  int32_t seg_x;               /* FUSED: planes marched per block (0 = tuned default) */
+  int32_t reserved1;
+} mplbm_config;
+
+/* FP64 reductions over fluid nodes (mplbm_reduce). */
+typedef struct mplbm_stats64 {
+  double mass;                 /* sum of all populations of the newest lattice */
+  double momentum[3];          /* sum of f_i e_i of the newest lattice */
+  double fluid_nodes;
+  double liquid_nodes;         /* rho > rhol_spinodal (from the last macro write) */
+  double vapor_nodes;          /* rho < rhog_spinodal */
+  double liquid_rho_sum, vapor_rho_sum;
+  double liquid_p_sum, vapor_p_sum;
+  double rho_min, rho_max, p_min, p_max;
+  double nonfinite;            /* populations that are NaN or Inf */
+} mplbm_stats64;
+
+/* Read-only facts about a context (mplbm_info). */
+typedef struct mplbm_info {
+  int64_t fluid_nodes;
+  int64_t cells;
+  int32_t storage, kernel;
+  int32_t tile_y, tile_z, seg_x;
+  int32_t bytes_per_pop;       /* storage bytes of one population */
+  double rho_ref, dev_scale;   /* deviation storage parameters in use */
+  int64_t device_bytes;        /* device memory held by the context */
+} mplbm_info;
+
+typedef struct mplbm_ctx mplbm_ctx;
+
+/* Step flags. */
+#define MPLBM_STEP_WRITE_MACRO 1u /* the last step of the call writes rho, p, u */
+
+/* Defaults of the reference constructor (src/multiphase_lbm.cu:35-80),
+ * domain 400^3 (include/lbm_model.cuh:4-6), FP32, AUTO kernel. */
+void mplbm_config_default(mplbm_config* cfg);
+
+/* Replaces transfer_parameter (src/iteration_host.cu:16-41) and the
+ * allocation half of MultiphaseLBM::resize_gpu (src/multiphase_lbm.cu:83-109):
+ * validates cfg, creates the stream and allocates both lattices and fields. */
+mplbm_status mplbm_create(const mplbm_config* cfg, mplbm_ctx** out);
+
+/* Frees device memory and the stream (src/multiphase_lbm.cu:1145-1160). */
+void mplbm_destroy(mplbm_ctx* ctx);
+
+/* Text of the last error of ctx, or of the last failed mplbm_create when
+ * ctx is NULL.  Valid until the next call on the same context. */
+const char* mplbm_last_error(const mplbm_ctx* ctx);
+
+/* Solid mask, host uint8 [x][y][z], nonzero = fluid (the reference's
+ * is_fluid, include/global_var.cuh:11).  Builds the per-node bounce-back
+ * link words on the GPU. */
+mplbm_status mplbm_set_geometry(mplbm_ctx* ctx, const uint8_t* is_fluid);
+
+/* Replaces multiphaselbm_initialization + multiphaselbm_init
+ * (src/iteration_host.cu:44-70, include/iteration_kernel.cuh:9-29):
+ * both lattices = t_i*rho, solid nodes take rho = rhos.  rho is host
+ * [x][y][z]; the _f64 variant keeps FP64 initial data exact. */
+mplbm_status mplbm_init_fields(mplbm_ctx* ctx, const float* rho);
+mplbm_status mplbm_init_fields_f64(mplbm_ctx* ctx, const double* rho);
+
+/* Replaces multiphaselbm_iteration (src/iteration_host.cu:73-81): runs
+ * nsteps steps numbered step, step+1, ... asynchronously on the context
+ * stream.  reverse and inject_type are the reference's injection controls.
+ * With MPLBM_STEP_WRITE_MACRO the last step also writes rho, p and the
+ * shifted velocity u to device fields (the reference writes rho and p
+ * every step, include/iteration_device.cuh:60-61; its host reads them only
+ * on statistics steps). */
+mplbm_status mplbm_step(mplbm_ctx* ctx, int64_t step, int32_t reverse, uint32_t inject_type,
+                        int32_t nsteps, uint32_t flags);
+
+/* As mplbm_step, and returns the time of the launched work measured with HIP
+ * events recorded on the context stream around it (synchronises). */
+mplbm_status mplbm_step_timed(mplbm_ctx* ctx, int64_t step, int32_t reverse,
+                              uint32_t inject_type, int32_t nsteps, uint32_t flags,
+                              double* elapsed_ms);
+
+/* Waits for the context stream. */
+mplbm_status mplbm_sync(mplbm_ctx* ctx);
+
+/* Copies the fields of the last macro write to host [x][y][z] arrays
+ * (any pointer may be NULL; u is [x][y][z][3]).  Replaces the reference
+ * host's unified-memory reads of rho_d/pressure_d
+ * (src/multiphase_lbm.cu:243-262,1061-1069).  Solid nodes read rho = rhos,
+ * p = 0, u = 0. */
+mplbm_status mplbm_read_fields(mplbm_ctx* ctx, float* rho, float* pressure, float* u);
+mplbm_status mplbm_read_fields_f64(mplbm_ctx* ctx, double* rho, double* pressure, double* u);
+
+/* Newest post-collision populations as host [x][y][z][19] doubles (decoded
+ * from deviation storage) and the inverse, for exact checkpoint/restart
+ * (the reference restarts from a 5-digit density VTK only,
+ * src/multiphase_lbm.cu:112-132). */
+mplbm_status mplbm_read_populations_f64(mplbm_ctx* ctx, double* f);
+mplbm_status mplbm_write_populations_f64(mplbm_ctx* ctx, const double* f);
+
+/* FP64 reductions of the newest lattice and of the last macro write
+ * (synchronises). */
+mplbm_status mplbm_reduce(mplbm_ctx* ctx, mplbm_stats64* out);
+
+/* Context facts (no synchronisation). */
+mplbm_status mplbm_info_get(const mplbm_ctx* ctx, mplbm_info* out);
+
+/* Library version string. */
+const char* mplbm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPLBM_H_ */

@@ -1,0 +1,1287 @@
+// libmplbm: MI355X (gfx950) D3Q19 Shan-Chen multiphase LBM step behind the
+// C ABI of include/mplbm.h.
+//
+// Device layout (structure of arrays): population i of cell c lives at
+// lattice[i * pstride + c], c = (x*ly + y)*lz + z, so a wavefront reading 64
+// consecutive z of one direction issues one coalesced 256-byte access.  Two
+// lattices (pull scheme, A -> B, pointer swap on the host instead of the
+// reference's do_swap kernel, include/iteration_kernel.cuh:44-48).  A 32-bit
+// link word per cell holds bit 0 = "cell is solid" and bit i = "upstream
+// node x - e_i is solid" (halfway bounce-back, iteration_kernel.cuh:112-115).
+//
+// Step kernels:
+//  * k_fused: one launch per step.  A workgroup owns a (y,z) tile with a
+//    one-cell ring and marches along x.  For plane x+1 it pulls all 19
+//    populations of every tile+ring cell, forms rho and psi and writes psi to
+//    a 4-slot LDS ring; after one barrier it evaluates the Shan-Chen force of
+//    plane x from LDS, collides the populations of plane x it kept in
+//    registers and stores them.  Each population is read from HBM once per
+//    step (ring re-reads hit L2/MALL) and written once.
+//  * k_density + k_collide: the two-pass schedule (psi of every node before
+//    any force), the plain form of the reference's do_collision without its
+//    psi race (iteration_device.cuh:62 vs :89-147).
+//
+// Arithmetic modes:
+//  * ModeF32: float storage, the reference's float arithmetic including its
+//    double-promoted sub-expressions (iteration_device.cuh:7-206).
+//  * ModeF64: double storage and arithmetic.
+//  * ModeH16 / ModeB16: 16-bit storage of g_i = (f_i/w_i - rho_ref)*scale,
+//    all arithmetic in float.
+// Built with -ffp-contract=off: no multiply-add is fused unless written.
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "d3q19.hpp"
+#include "mplbm.h"
+
+namespace mplbm {
+
+// ----------------------------------------------------------------------------
+// storage / arithmetic modes
+
+struct ModeF32 {
+  using S = float;
+  using C = float;
+  using M = float;  // macro-field type
+  static constexpr bool kRef = true;
+  static constexpr int kStorage = MPLBM_FP32;
+};
+struct ModeF64 {
+  using S = double;
+  using C = double;
+  using M = double;
+  static constexpr bool kRef = true;
+  static constexpr int kStorage = MPLBM_FP64;
+};
+struct ModeH16 {
+  using S = __half;
+  using C = float;
+  using M = float;
+  static constexpr bool kRef = false;
+  static constexpr int kStorage = MPLBM_FP16S;
+};
+struct ModeB16 {
+  using S = __hip_bfloat16;
+  using C = float;
+  using M = float;
+  static constexpr bool kRef = false;
+  static constexpr int kStorage = MPLBM_BF16S;
+};
+
+// Per-step physics, converted on the host to the arithmetic type C.
+template <class C>
+struct Phys {
+  Eos<C> eos;
+  C tau;            // (C)tau
+  C inv_tau;        // (C)(1.0/(double)tau), iteration_device.cuh:165
+  C neg_G;          // -G
+  C rhos;           // solid density
+  C rhol_sp;        // liquid spinodal threshold
+  C inc;            // rho_increment * reverse
+  C inject_ylim;    // inject_position * ly
+  C g[3];           // body acceleration
+  float rho_ref, scale, inv_scale;  // deviation storage
+  int inject_now;   // step % rho_inject_period == 0
+  int inject_type;
+  int has_body;
+  int any_face;
+  int face[6];      // face types
+  C face_rho[6];
+  int write_macro;
+};
+
+// ----------------------------------------------------------------------------
+// population encode / decode
+
+template <class Mo>
+__device__ __forceinline__ typename Mo::C decode(typename Mo::S s, int i, const Phys<typename Mo::C>& ph) {
+  if constexpr (Mo::kRef) {
+    return s;
+  } else {
+    float g;
+    if constexpr (Mo::kStorage == MPLBM_FP16S) g = __half2float(s);
+    else g = __bfloat162float(s);
+    return weight<float>(i) * (g * ph.inv_scale + ph.rho_ref);
+  }
+}
+
+template <class Mo>
+__device__ __forceinline__ typename Mo::S encode(typename Mo::C v, int i, const Phys<typename Mo::C>& ph) {
+  if constexpr (Mo::kRef) {
+    return v;
+  } else {
+    const float g = (v * inv_weight_f(i) - ph.rho_ref) * ph.scale;
+    if constexpr (Mo::kStorage == MPLBM_FP16S) return __float2half_rn(g);
+    else return __float2bfloat16(g);
+  }
+}
+
+// ----------------------------------------------------------------------------
+// per-cell physics
+
+// Density with injection (iteration_device.cuh:40-59): rho = sum_i f_i in
+// ascending i, then the injection rule keyed on y.
+template <class C>
+__device__ __forceinline__ C density(const C* f, int gy, const Phys<C>& ph) {
+  C d = f[0];
+#pragma unroll
+  for (int i = 1; i < kQ; ++i) d = d + f[i];
+  if (ph.inject_now && (C)gy < ph.inject_ylim) {
+    if (ph.inject_type == 0) d = d + ph.inc;
+    else if (ph.inject_type == 1) { if (d < ph.rhol_sp) d = d + ph.inc; }
+    else if (ph.inject_type == 2) { if (d > ph.rhol_sp) d = d + ph.inc; }
+  }
+  return d;
+}
+
+template <class Mo>
+__device__ __forceinline__ typename Mo::C psi_of(const Phys<typename Mo::C>& ph, typename Mo::C rho) {
+  if constexpr (Mo::kRef) {
+    return eos_psi(ph.eos, rho);
+  } else {
+    if (ph.eos.carn_star) return eos_psi(ph.eos, rho);
+    return 1.0f - __expf(-rho);
+  }
+}
+
+template <class Mo>
+__device__ __forceinline__ typename Mo::C pressure_of(const Phys<typename Mo::C>& ph, typename Mo::C rho) {
+  if constexpr (Mo::kRef) {
+    return eos_pressure(ph.eos, rho);
+  } else {
+    if (ph.eos.carn_star) return eos_pressure(ph.eos, rho);
+    const float psi = 1.0f - __expf(-rho);
+    return ph.eos.cs2 * rho + ph.eos.half_cs2G * (psi * psi);
+  }
+}
+
+// Shan-Chen force from the 18 neighbour psi values (psi_nb[i] = psi(x+e_i)),
+// iteration_device.cuh:89-147: F_a = (-G psi(x)) * sum_i (w_i psi_nb[i]) e_ia
+// with the non-zero terms added left to right in ascending i (the zero terms
+// of the reference only add signed zeros).
+template <class C>
+__device__ __forceinline__ void sc_force(const C* psi_nb, C psi_c, const Phys<C>& ph, C* F) {
+  C s[3];
+  bool started[3] = {false, false, false};
+#pragma unroll
+  for (int i = 1; i < kQ; ++i) {
+    const C wp = weight<C>(i) * psi_nb[i];
+    const int e[3] = {ex(i), ey(i), ez(i)};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (e[a] == 0) continue;
+      const C t = e[a] > 0 ? wp : -wp;
+      s[a] = started[a] ? s[a] + t : t;
+      started[a] = true;
+    }
+  }
+  const C gpsi = ph.neg_G * psi_c;
+  F[0] = gpsi * s[0];
+  F[1] = gpsi * s[1];
+  F[2] = gpsi * s[2];
+}
+
+// Velocity shift and BGK (iteration_device.cuh:162-206) in place on f.
+// u is the shifted velocity (m + tau F) / rho that the reference builds.
+template <class Mo>
+__device__ __forceinline__ void collide(typename Mo::C* f, typename Mo::C rho, typename Mo::C* F,
+                                        const Phys<typename Mo::C>& ph, typename Mo::C* u) {
+  using C = typename Mo::C;
+  if (ph.has_body) {
+    F[0] = F[0] + rho * ph.g[0];
+    F[1] = F[1] + rho * ph.g[1];
+    F[2] = F[2] + rho * ph.g[2];
+  }
+  C m[3];
+  bool started[3] = {false, false, false};
+#pragma unroll
+  for (int i = 1; i < kQ; ++i) {
+    const int e[3] = {ex(i), ey(i), ez(i)};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (e[a] == 0) continue;
+      const C t = e[a] > 0 ? f[i] : -f[i];
+      m[a] = started[a] ? m[a] + t : t;
+      started[a] = true;
+    }
+  }
+  C ux = m[0] + ph.tau * F[0];
+  C uy = m[1] + ph.tau * F[1];
+  C uz = m[2] + ph.tau * F[2];
+  if constexpr (Mo::kRef) {
+    const C inv_rho = (C)(1.0 / (double)rho);
+    ux = ux * inv_rho;
+    uy = uy * inv_rho;
+    uz = uz * inv_rho;
+    const C usq = ux * ux + uy * uy + uz * uz;
+    const double usq15 = 1.5 * (double)usq;
+    const double itau = (double)ph.inv_tau;
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) {
+      C eu;
+      {
+        const int e[3] = {ex(i), ey(i), ez(i)};
+        const C uu[3] = {ux, uy, uz};
+        bool st = false;
+        eu = (C)0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          if (e[a] == 0) continue;
+          const C t = e[a] > 0 ? uu[a] : -uu[a];
+          eu = st ? eu + t : t;
+          st = true;
+        }
+      }
+      const C trho = weight<C>(i) * rho;
+      const double deu = (double)eu;
+      const double poly = 1.0 + 3.0 * deu + 4.5 * deu * deu - usq15;
+      const double fi = (double)f[i];
+      f[i] = (C)(fi + ((double)trho * poly - fi) * itau);
+    }
+  } else {
+    const float inv_rho = 1.0f / rho;
+    ux *= inv_rho;
+    uy *= inv_rho;
+    uz *= inv_rho;
+    const float usq15 = 1.5f * (ux * ux + uy * uy + uz * uz);
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) {
+      const float eu = (float)ex(i) * ux + (float)ey(i) * uy + (float)ez(i) * uz;
+      const float feq = weight<float>(i) * rho * (1.0f + 3.0f * eu + 4.5f * eu * eu - usq15);
+      f[i] = f[i] + (feq - f[i]) * ph.inv_tau;
+    }
+  }
+  u[0] = ux;
+  u[1] = uy;
+  u[2] = uz;
+}
+
+// Anti-bounce-back at a PRESSURE face for direction i entering the domain:
+// B_i = -A_opp(i) + 2 t_i rho_w (1 + 4.5 (e_i.u)^2 - 1.5 u^2) with
+// u = sum(A e)/sum(A) of the face node (iteration_device.cuh:213-243,282-286).
+template <class Mo>
+__device__ __forceinline__ typename Mo::C abb_value(const typename Mo::C* a, int i, typename Mo::C rho_w) {
+  using C = typename Mo::C;
+  C ra = (C)0, mx = (C)0, my = (C)0, mz = (C)0;
+#pragma unroll
+  for (int k = 0; k < kQ; ++k) {
+    ra = ra + a[k];
+    mx = mx + a[k] * (C)ex(k);
+    my = my + a[k] * (C)ey(k);
+    mz = mz + a[k] * (C)ez(k);
+  }
+  const C inv = (C)(1.0 / (double)ra);
+  const C ux = mx * inv, uy = my * inv, uz = mz * inv;
+  const C usq = ux * ux + uy * uy + uz * uz;
+  const C eu = (C)ex(i) * ux + (C)ey(i) * uy + (C)ez(i) * uz;
+  const C added = (C)((double)((C)2 * weight<C>(i) * rho_w) *
+                      (1.0 + 4.5 * (double)eu * (double)eu - 1.5 * (double)usq));
+  return -a[opp(i)] + added;
+}
+
+// Which non-periodic face (0..5) direction i enters through at (x,y,z), or -1.
+// Later faces win, as the reference applies x, y, z face rules in turn.
+template <class C>
+__device__ __forceinline__ int entering_face(const Phys<C>& ph, int i, int x, int y, int z, int lx, int ly,
+                                             int lz) {
+  int f = -1;
+  if (ph.face[0] && x == 0 && ex(i) == 1) f = 0;
+  if (ph.face[1] && x == lx - 1 && ex(i) == -1) f = 1;
+  if (ph.face[2] && y == 0 && ey(i) == 1) f = 2;
+  if (ph.face[3] && y == ly - 1 && ey(i) == -1) f = 3;
+  if (ph.face[4] && z == 0 && ez(i) == 1) f = 4;
+  if (ph.face[5] && z == lz - 1 && ez(i) == -1) f = 5;
+  return f;
+}
+
+// The force is not evaluated on a node lying on the boundary planes of a
+// non-periodic axis (iteration_device.cuh:79-87); F = 0 there.
+template <class C>
+__device__ __forceinline__ bool force_skipped(const Phys<C>& ph, int x, int y, int z, int lx, int ly, int lz) {
+  if ((ph.face[0] | ph.face[1]) && (x == 0 || x == lx - 1)) return true;
+  if ((ph.face[2] | ph.face[3]) && (y == 0 || y == ly - 1)) return true;
+  if ((ph.face[4] | ph.face[5]) && (z == 0 || z == lz - 1)) return true;
+  return false;
+}
+
+// ----------------------------------------------------------------------------
+// kernel arguments
+
+template <class Mo>
+struct Lattice {
+  const typename Mo::S* fin;
+  typename Mo::S* fout;
+  const uint32_t* link;
+  typename Mo::M* rho;   // macro fields (written when ph.write_macro)
+  typename Mo::M* pres;
+  typename Mo::M* u;     // 3 planes of ncell
+  typename Mo::C* psi;   // two-pass psi field
+  uint64_t pstride;      // elements between population arrays
+  int lx, ly, lz;
+  uint32_t plane;        // ly*lz
+  uint32_t ncell;
+};
+
+struct Tiling {
+  int TY, TZ;            // threads per tile in y and z (ring included unless wrapped)
+  int tiles_y, tiles_z;
+  int seg, nseg;         // planes per block, segments along x
+  int wrap_y, wrap_z;    // 1: the whole axis lies in one tile, neighbours wrap in LDS
+  int nblocks;           // real blocks (grid may be padded to a multiple of 8)
+  int xcd_remap;
+};
+
+// Pull the 19 post-stream populations of one fluid cell (stream + boundary,
+// iteration_kernel.cuh:51-118).  `src(i)` is the upstream cell index for
+// direction i, `self` the cell's own index.
+template <class Mo, class SrcFn>
+__device__ __forceinline__ void pull(const Lattice<Mo>& L, const Phys<typename Mo::C>& ph, uint32_t lk,
+                                     uint32_t self, SrcFn src, typename Mo::C* f) {
+  f[0] = decode<Mo>(L.fin[self], 0, ph);
+#pragma unroll
+  for (int i = 1; i < kQ; ++i) {
+    const bool bb = (lk >> i) & 1u;
+    const typename Mo::S* p = bb ? (L.fin + (uint64_t)opp(i) * L.pstride + self)
+                                 : (L.fin + (uint64_t)i * L.pstride + src(i));
+    // a bounce-back direction reads population opp(i) of this cell but
+    // decodes with the weight of opp(i), which equals the weight of i
+    f[i] = decode<Mo>(*p, i, ph);
+  }
+}
+
+// Face rule applied after the pull on a face cell (rare, divergent path).
+template <class Mo>
+__device__ __noinline__ void apply_faces(const Lattice<Mo>& L, const Phys<typename Mo::C>& ph, uint32_t self,
+                                         int x, int y, int z, typename Mo::C* f) {
+  using C = typename Mo::C;
+  C a[kQ];
+#pragma unroll
+  for (int k = 0; k < kQ; ++k) a[k] = decode<Mo>(L.fin[(uint64_t)k * L.pstride + self], k, ph);
+#pragma unroll
+  for (int i = 1; i < kQ; ++i) {
+    const int fc = entering_face(ph, i, x, y, z, L.lx, L.ly, L.lz);
+    if (fc >= 0) f[i] = abb_value<Mo>(a, i, ph.face_rho[fc]);
+  }
+}
+
+template <class C>
+__device__ __forceinline__ bool on_face(const Phys<C>& ph, int x, int y, int z, int lx, int ly, int lz) {
+  return (ph.face[0] && x == 0) || (ph.face[1] && x == lx - 1) || (ph.face[2] && y == 0) ||
+         (ph.face[3] && y == ly - 1) || (ph.face[4] && z == 0) || (ph.face[5] && z == lz - 1);
+}
+
+__device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+// ----------------------------------------------------------------------------
+// fused marching stream-collide kernel
+
+template <class Mo>
+__global__ __launch_bounds__(1024) void k_fused(Lattice<Mo> L, Phys<typename Mo::C> ph, Tiling T) {
+  using C = typename Mo::C;
+  extern __shared__ unsigned char smem_raw[];
+  C* lds = reinterpret_cast<C*>(smem_raw);
+
+  int b = blockIdx.x;
+  if (T.xcd_remap) {
+    // consecutive logical tiles share an XCD (and its L2) so the ring
+    // re-reads of neighbouring tiles hit cache
+    const int per = gridDim.x >> 3;
+    b = (b & 7) * per + (b >> 3);
+  }
+  if (b >= T.nblocks) return;  // whole padding block leaves before any barrier
+  const int ntile = T.tiles_y * T.tiles_z;
+  const int seg = b / ntile;
+  const int t = b - seg * ntile;
+  const int tyi = t / T.tiles_z;
+  const int tzi = t - tyi * T.tiles_z;
+
+  const int TY = T.TY, TZ = T.TZ, TT = TY * TZ;
+  const int tid = threadIdx.x;
+  const bool thr = tid < TT;
+  const int ty = thr ? tid / TZ : 0;
+  const int tz = thr ? tid - ty * TZ : 0;
+
+  int gy, gz, rowm, rowp, colm, colp;
+  bool iny, inz;
+  if (T.wrap_y) {
+    gy = ty;
+    iny = thr;
+    rowm = ty == 0 ? L.ly - 1 : ty - 1;
+    rowp = ty + 1 == L.ly ? 0 : ty + 1;
+  } else {
+    const int yy = tyi * (TY - 2) - 1 + ty;
+    gy = wrapi(yy, L.ly);
+    iny = thr && ty >= 1 && ty <= TY - 2 && yy < L.ly;
+    rowm = ty - 1;
+    rowp = ty + 1;
+  }
+  if (T.wrap_z) {
+    gz = tz;
+    inz = thr;
+    colm = tz == 0 ? L.lz - 1 : tz - 1;
+    colp = tz + 1 == L.lz ? 0 : tz + 1;
+  } else {
+    const int zz = tzi * (TZ - 2) - 1 + tz;
+    gz = wrapi(zz, L.lz);
+    inz = thr && tz >= 1 && tz <= TZ - 2 && zz < L.lz;
+    colm = tz - 1;
+    colp = tz + 1;
+  }
+  const bool interior = iny && inz;
+
+  // (y,z) offsets of the 9 in-plane neighbours, index [dy+1][dz+1]
+  uint32_t yzo[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const uint32_t row = (uint32_t)wrapi(gy + a - 1, L.ly) * (uint32_t)L.lz;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) yzo[a][c] = row + (uint32_t)wrapi(gz + c - 1, L.lz);
+  }
+  const int lrow[3] = {rowm, ty, rowp};
+  const int lcol[3] = {colm, tz, colp};
+
+  const C psi_solid = psi_of<Mo>(ph, ph.rhos);
+  const int x0 = seg * T.seg;
+  const int x1 = min(x0 + T.seg, L.lx);
+
+  // pulls plane xp into f, returns rho (meaningless for solid cells) and link
+  auto load_plane = [&](int xp, C* f, uint32_t& lk) -> C {
+    const int xm = xp == 0 ? L.lx - 1 : xp - 1;
+    const int xq = xp + 1 == L.lx ? 0 : xp + 1;
+    const uint32_t p0 = (uint32_t)xp * L.plane, pm = (uint32_t)xm * L.plane, pq = (uint32_t)xq * L.plane;
+    const uint32_t self = p0 + yzo[1][1];
+    lk = thr ? L.link[self] : 1u;
+    C r = (C)0;
+    if (!(lk & 1u)) {
+      pull<Mo>(L, ph, lk, self,
+               [&](int i) -> uint32_t {
+                 const uint32_t pl = ex(i) == -1 ? pq : (ex(i) == 1 ? pm : p0);
+                 return pl + yzo[1 - ey(i)][1 - ez(i)];
+               },
+               f);
+      if (ph.any_face && on_face(ph, xp, gy, gz, L.lx, L.ly, L.lz)) apply_faces<Mo>(L, ph, self, xp, gy, gz, f);
+      r = density(f, gy, ph);
+    }
+    return r;
+  };
+
+  C P[kQ], N[kQ];
+  C rhoP, rhoN;
+  uint32_t lkP, lkN;
+  const int myl = ty * TZ + tz;
+
+  {  // warm-up: plane x0-1 (psi only) and plane x0 (kept)
+    const int xw = x0 == 0 ? L.lx - 1 : x0 - 1;
+    const C r = load_plane(xw, N, lkN);
+    if (thr) lds[0 * TT + myl] = (lkN & 1u) ? psi_solid : psi_of<Mo>(ph, r);
+    rhoP = load_plane(x0, P, lkP);
+    if (thr) lds[1 * TT + myl] = (lkP & 1u) ? psi_solid : psi_of<Mo>(ph, rhoP);
+  }
+
+  for (int j = 0; j < x1 - x0; ++j) {
+    const int x = x0 + j;
+    const int xn = x + 1 == L.lx ? 0 : x + 1;
+    rhoN = load_plane(xn, N, lkN);
+    if (thr) lds[((j + 2) & 3) * TT + myl] = (lkN & 1u) ? psi_solid : psi_of<Mo>(ph, rhoN);
+    __syncthreads();
+
+    if (interior && !(lkP & 1u)) {
+      const uint32_t self = (uint32_t)x * L.plane + yzo[1][1];
+      C F[3];
+      if (ph.any_face && force_skipped(ph, x, gy, gz, L.lx, L.ly, L.lz)) {
+        F[0] = F[1] = F[2] = (C)0;
+      } else {
+        C pn[kQ];
+        pn[0] = (C)0;
+#pragma unroll
+        for (int i = 1; i < kQ; ++i) {
+          const int slot = (j + 1 + ex(i)) & 3;
+          pn[i] = lds[slot * TT + lrow[1 + ey(i)] * TZ + lcol[1 + ez(i)]];
+        }
+        sc_force(pn, lds[((j + 1) & 3) * TT + myl], ph, F);
+      }
+      C u[3];
+      collide<Mo>(P, rhoP, F, ph, u);
+#pragma unroll
+      for (int i = 0; i < kQ; ++i) L.fout[(uint64_t)i * L.pstride + self] = encode<Mo>(P[i], i, ph);
+      if (ph.write_macro) {
+        L.rho[self] = (typename Mo::M)rhoP;
+        L.pres[self] = (typename Mo::M)pressure_of<Mo>(ph, rhoP);
+        L.u[self] = (typename Mo::M)u[0];
+        L.u[(uint64_t)L.ncell + self] = (typename Mo::M)u[1];
+        L.u[2ull * L.ncell + self] = (typename Mo::M)u[2];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) P[i] = N[i];
+    rhoP = rhoN;
+    lkP = lkN;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// two-pass kernels (one thread per cell)
+
+template <class Mo>
+__device__ __forceinline__ void cell_coords(const Lattice<Mo>& L, uint32_t c, int& x, int& y, int& z) {
+  x = (int)(c / L.plane);
+  const uint32_t r = c - (uint32_t)x * L.plane;
+  y = (int)(r / (uint32_t)L.lz);
+  z = (int)(r - (uint32_t)y * (uint32_t)L.lz);
+}
+
+template <class Mo>
+__device__ __forceinline__ uint32_t cell_at(const Lattice<Mo>& L, int x, int y, int z) {
+  return ((uint32_t)wrapi(x, L.lx) * (uint32_t)L.ly + (uint32_t)wrapi(y, L.ly)) * (uint32_t)L.lz +
+         (uint32_t)wrapi(z, L.lz);
+}
+
+template <class Mo>
+__device__ __forceinline__ typename Mo::C pull_cell(const Lattice<Mo>& L, const Phys<typename Mo::C>& ph,
+                                                   uint32_t c, uint32_t lk, int x, int y, int z,
+                                                   typename Mo::C* f) {
+  pull<Mo>(L, ph, lk, c, [&](int i) -> uint32_t { return cell_at(L, x - ex(i), y - ey(i), z - ez(i)); }, f);
+  if (ph.any_face && on_face(ph, x, y, z, L.lx, L.ly, L.lz)) apply_faces<Mo>(L, ph, c, x, y, z, f);
+  return density(f, y, ph);
+}
+
+template <class Mo>
+__global__ __launch_bounds__(256) void k_density(Lattice<Mo> L, Phys<typename Mo::C> ph) {
+  using C = typename Mo::C;
+  const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+  if (c >= L.ncell) return;
+  const uint32_t lk = L.link[c];
+  if (lk & 1u) return;
+  int x, y, z;
+  cell_coords(L, c, x, y, z);
+  C f[kQ];
+  const C rho = pull_cell(L, ph, c, lk, x, y, z, f);
+  L.psi[c] = psi_of<Mo>(ph, rho);
+  if (ph.write_macro) {
+    L.rho[c] = (typename Mo::M)rho;
+    L.pres[c] = (typename Mo::M)pressure_of<Mo>(ph, rho);
+  }
+}
+
+template <class Mo>
+__global__ __launch_bounds__(256) void k_collide(Lattice<Mo> L, Phys<typename Mo::C> ph) {
+  using C = typename Mo::C;
+  const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+  if (c >= L.ncell) return;
+  const uint32_t lk = L.link[c];
+  if (lk & 1u) return;
+  int x, y, z;
+  cell_coords(L, c, x, y, z);
+  C f[kQ];
+  const C rho = pull_cell(L, ph, c, lk, x, y, z, f);
+  C F[3];
+  if (ph.any_face && force_skipped(ph, x, y, z, L.lx, L.ly, L.lz)) {
+    F[0] = F[1] = F[2] = (C)0;
+  } else {
+    C pn[kQ];
+    pn[0] = (C)0;
+#pragma unroll
+    for (int i = 1; i < kQ; ++i) pn[i] = L.psi[cell_at(L, x + ex(i), y + ey(i), z + ez(i))];
+    sc_force(pn, L.psi[c], ph, F);
+  }
+  C u[3];
+  collide<Mo>(f, rho, F, ph, u);
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) L.fout[(uint64_t)i * L.pstride + c] = encode<Mo>(f[i], i, ph);
+  if (ph.write_macro) {
+    L.u[c] = (typename Mo::M)u[0];
+    L.u[(uint64_t)L.ncell + c] = (typename Mo::M)u[1];
+    L.u[2ull * L.ncell + c] = (typename Mo::M)u[2];
+  }
+}
+
+// ----------------------------------------------------------------------------
+// set-up kernels
+
+// link word: bit 0 solid cell, bit i upstream node x - e_i solid
+__global__ void k_links(const uint8_t* __restrict__ fluid, uint32_t* __restrict__ link, int lx, int ly, int lz) {
+  const uint32_t n = (uint32_t)lx * ly * lz;
+  const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+  if (c >= n) return;
+  const uint32_t plane = (uint32_t)ly * lz;
+  const int x = (int)(c / plane);
+  const uint32_t r = c - (uint32_t)x * plane;
+  const int y = (int)(r / (uint32_t)lz);
+  const int z = (int)(r - (uint32_t)y * lz);
+  uint32_t w = fluid[c] ? 0u : 1u;
+#pragma unroll
+  for (int i = 1; i < kQ; ++i) {
+    const uint32_t u = ((uint32_t)wrapi(x - ex(i), lx) * ly + (uint32_t)wrapi(y - ey(i), ly)) * lz +
+                       (uint32_t)wrapi(z - ez(i), lz);
+    if (!fluid[u]) w |= 1u << i;
+  }
+  link[c] = w;
+}
+
+// multiphaselbm_init (include/iteration_kernel.cuh:9-29)
+template <class Mo>
+__global__ void k_init(Lattice<Mo> L, typename Mo::S* fother, Phys<typename Mo::C> ph, const double* rho_in) {
+  using C = typename Mo::C;
+  const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+  if (c >= L.ncell) return;
+  const C rho = (C)rho_in[c];
+  const bool fluid = !(L.link[c] & 1u);
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) {
+    const typename Mo::S v = encode<Mo>(weight<C>(i) * rho, i, ph);
+    L.fout[(uint64_t)i * L.pstride + c] = v;
+    fother[(uint64_t)i * L.pstride + c] = v;
+  }
+  if (fluid) {
+    L.rho[c] = (typename Mo::M)rho;
+    L.pres[c] = (typename Mo::M)pressure_of<Mo>(ph, rho);
+    L.psi[c] = psi_of<Mo>(ph, rho);
+  } else {
+    L.rho[c] = (typename Mo::M)ph.rhos;
+    L.pres[c] = (typename Mo::M)0;
+    L.psi[c] = psi_of<Mo>(ph, ph.rhos);
+  }
+  L.u[c] = 0;
+  L.u[(uint64_t)L.ncell + c] = 0;
+  L.u[2ull * L.ncell + c] = 0;
+}
+
+// decode the current lattice into [cell][19] doubles / encode back
+template <class Mo>
+__global__ void k_read_pops(Lattice<Mo> L, Phys<typename Mo::C> ph, double* out) {
+  const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+  if (c >= L.ncell) return;
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) out[(uint64_t)c * kQ + i] = (double)decode<Mo>(L.fin[(uint64_t)i * L.pstride + c], i, ph);
+}
+
+template <class Mo>
+__global__ void k_write_pops(Lattice<Mo> L, Phys<typename Mo::C> ph, const double* in) {
+  using C = typename Mo::C;
+  const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+  if (c >= L.ncell) return;
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) L.fout[(uint64_t)i * L.pstride + c] = encode<Mo>((C)in[(uint64_t)c * kQ + i], i, ph);
+}
+
+// ----------------------------------------------------------------------------
+// FP64 reductions: per-block partials, then one block folds them in order
+
+constexpr int kRedFields = 16;  // see mplbm_stats64 order
+constexpr int kRedThreads = 256;
+
+template <class Mo>
+__global__ __launch_bounds__(kRedThreads) void k_reduce(Lattice<Mo> L, Phys<typename Mo::C> ph, double rhol,
+                                                        double rhog, double* partial) {
+  __shared__ double sh[kRedThreads];
+  double acc[kRedFields];
+#pragma unroll
+  for (int k = 0; k < kRedFields; ++k) acc[k] = 0.0;
+  acc[11] = INFINITY; acc[12] = -INFINITY; acc[13] = INFINITY; acc[14] = -INFINITY;
+  for (uint32_t c = blockIdx.x * kRedThreads + threadIdx.x; c < L.ncell; c += gridDim.x * kRedThreads) {
+    if (L.link[c] & 1u) continue;
+    double s = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) {
+      const double f = (double)decode<Mo>(L.fin[(uint64_t)i * L.pstride + c], i, ph);
+      if (!(f - f == 0.0)) acc[15] += 1.0;
+      s += f;
+      mx += f * ex(i);
+      my += f * ey(i);
+      mz += f * ez(i);
+    }
+    acc[0] += s; acc[1] += mx; acc[2] += my; acc[3] += mz;
+    acc[4] += 1.0;
+    const double r = (double)L.rho[c], p = (double)L.pres[c];
+    if (r > rhol) { acc[5] += 1.0; acc[7] += r; acc[9] += p; }
+    else if (r < rhog) { acc[6] += 1.0; acc[8] += r; acc[10] += p; }
+    acc[11] = fmin(acc[11], r); acc[12] = fmax(acc[12], r);
+    acc[13] = fmin(acc[13], p); acc[14] = fmax(acc[14], p);
+  }
+  for (int k = 0; k < kRedFields; ++k) {
+    sh[threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) {
+        const double a = sh[threadIdx.x], bb = sh[threadIdx.x + s];
+        sh[threadIdx.x] = (k >= 11 && k <= 14) ? ((k & 1) ? fmin(a, bb) : fmax(a, bb)) : a + bb;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(uint64_t)blockIdx.x * kRedFields + k] = sh[0];
+    __syncthreads();
+  }
+}
+
+__global__ void k_reduce_final(const double* partial, int nblocks, double* out) {
+  const int k = threadIdx.x;
+  if (k >= kRedFields) return;
+  double a = partial[k];
+  for (int b = 1; b < nblocks; ++b) {
+    const double v = partial[(uint64_t)b * kRedFields + k];
+    a = (k >= 11 && k <= 14) ? ((k & 1) ? fmin(a, v) : fmax(a, v)) : a + v;
+  }
+  out[k] = a;
+}
+
+}  // namespace mplbm
+
+// ============================================================================
+// host side
+
+using namespace mplbm;
+
+struct mplbm_ctx {
+  mplbm_config cfg{};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  uint64_t ncell = 0, pstride = 0;
+  int bytes_pop = 4;
+  void* lat[2] = {nullptr, nullptr};  // population lattices
+  int cur = 0;                        // lattice holding the newest post-collision state
+  uint32_t* link = nullptr;
+  uint8_t* mask = nullptr;
+  void* rho = nullptr;
+  void* pres = nullptr;
+  void* u = nullptr;
+  void* psi = nullptr;
+  double* red_partial = nullptr;
+  double* red_out = nullptr;
+  int red_blocks = 0;
+  bool have_geometry = false, have_fields = false;
+  int64_t fluid_nodes = 0;
+  int64_t device_bytes = 0;
+  Tiling tiling{};
+  int kernel = MPLBM_KERNEL_FUSED;
+  float rho_ref = 0.f, scale = 1.f;
+  std::string err;
+};
+
+static std::string g_create_error;
+
+static mplbm_status fail(mplbm_ctx* ctx, mplbm_status st, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->err = buf;
+  else g_create_error = buf;
+  return st;
+}
+
+#define HIP_TRY(ctx, call)                                                                      \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess)                                                                       \
+      return fail((ctx), e_ == hipErrorOutOfMemory ? MPLBM_ERR_NOMEM : MPLBM_ERR_HIP, "%s: %s (%s:%d)", \
+                  #call, hipGetErrorString(e_), __FILE__, __LINE__);                            \
+  } while (0)
+
+static size_t storage_bytes(int storage) {
+  switch (storage) {
+    case MPLBM_FP32: return 4;
+    case MPLBM_FP64: return 8;
+    default: return 2;
+  }
+}
+
+template <class C>
+static Phys<C> make_phys(const mplbm_ctx* ctx, int64_t step, int32_t reverse, uint32_t inject_type,
+                         bool write_macro) {
+  const mplbm_config& c = ctx->cfg;
+  Phys<C> ph{};
+  ph.eos.G = (C)c.G;
+  ph.eos.cs2 = (C)(1.0 / 3.0);
+  ph.eos.half_cs2G = ph.eos.cs2 * ph.eos.G / (C)2;
+  ph.eos.T = (C)c.T_Tc * (C)0.0943;
+  ph.eos.carn_star = c.carn_star != 0;
+  ph.tau = (C)c.tau;
+  ph.inv_tau = (C)(1.0 / (double)ph.tau);
+  ph.neg_G = -ph.eos.G;
+  ph.rhos = (C)c.rhos;
+  ph.rhol_sp = (C)c.rhol_spinodal;
+  ph.inc = (C)c.rho_increment * (C)reverse;
+  ph.inject_ylim = (C)c.inject_position * (C)c.ly;
+  ph.inject_now = c.rho_inject_period > 0 && ((uint64_t)step % (uint64_t)c.rho_inject_period) == 0;
+  ph.inject_type = (int)inject_type;
+  ph.has_body = c.body_force[0] != 0 || c.body_force[1] != 0 || c.body_force[2] != 0;
+  for (int a = 0; a < 3; ++a) ph.g[a] = (C)c.body_force[a];
+  ph.any_face = 0;
+  for (int f = 0; f < 6; ++f) {
+    ph.face[f] = c.face_type[f];
+    ph.face_rho[f] = (C)c.face_rho[f];
+    ph.any_face |= c.face_type[f] != 0;
+  }
+  ph.rho_ref = ctx->rho_ref;
+  ph.scale = ctx->scale;
+  ph.inv_scale = 1.0f / ctx->scale;
+  ph.write_macro = write_macro ? 1 : 0;
+  return ph;
+}
+
+template <class Mo>
+static Lattice<Mo> make_lattice(const mplbm_ctx* ctx, int in, int out) {
+  Lattice<Mo> L{};
+  L.fin = static_cast<const typename Mo::S*>(ctx->lat[in]);
+  L.fout = static_cast<typename Mo::S*>(ctx->lat[out]);
+  L.link = ctx->link;
+  L.rho = static_cast<typename Mo::M*>(ctx->rho);
+  L.pres = static_cast<typename Mo::M*>(ctx->pres);
+  L.u = static_cast<typename Mo::M*>(ctx->u);
+  L.psi = static_cast<typename Mo::C*>(ctx->psi);
+  L.pstride = ctx->pstride;
+  L.lx = (int)ctx->cfg.lx;
+  L.ly = (int)ctx->cfg.ly;
+  L.lz = (int)ctx->cfg.lz;
+  L.plane = (uint32_t)(ctx->cfg.ly * ctx->cfg.lz);
+  L.ncell = (uint32_t)ctx->ncell;
+  return L;
+}
+
+// Tile choice for the fused kernel: a whole axis goes into one tile when it
+// fits (no ring, neighbours wrap inside LDS); otherwise a ring of one cell.
+static mplbm_status choose_tiling(mplbm_ctx* ctx) {
+  const mplbm_config& c = ctx->cfg;
+  Tiling T{};
+  int tz = c.tile_z, ty = c.tile_y, seg = c.seg_x;
+  if (tz <= 0) tz = c.lz <= 64 ? (int)c.lz : 34;
+  if (tz >= c.lz) { T.wrap_z = 1; tz = (int)c.lz; }
+  if (ty <= 0) {
+    ty = std::max(1, 1024 / tz);
+    if (ty > 32) ty = 32;
+  }
+  if (ty >= c.ly && (int64_t)tz * c.ly <= 1024) { T.wrap_y = 1; ty = (int)c.ly; }
+  if (ty > c.ly) ty = (int)c.ly;
+  if (!T.wrap_z && tz < 3) return fail(ctx, MPLBM_ERR_ARG, "tile_z %d too small", tz);
+  if (!T.wrap_y && ty < 3) return fail(ctx, MPLBM_ERR_ARG, "tile_y %d too small", ty);
+  if ((int64_t)ty * tz > 1024) return fail(ctx, MPLBM_ERR_ARG, "tile %dx%d exceeds 1024 threads", ty, tz);
+  T.TY = ty;
+  T.TZ = tz;
+  T.tiles_y = T.wrap_y ? 1 : (int)((c.ly + (ty - 2) - 1) / (ty - 2));
+  T.tiles_z = T.wrap_z ? 1 : (int)((c.lz + (tz - 2) - 1) / (tz - 2));
+  if (seg <= 0) seg = 16;
+  if (seg > c.lx) seg = (int)c.lx;
+  T.seg = seg;
+  T.nseg = (int)((c.lx + seg - 1) / seg);
+  const int64_t nb = (int64_t)T.tiles_y * T.tiles_z * T.nseg;
+  if (nb > (1LL << 30)) return fail(ctx, MPLBM_ERR_ARG, "too many blocks");
+  T.nblocks = (int)nb;
+  T.xcd_remap = 1;
+  ctx->tiling = T;
+  return MPLBM_OK;
+}
+
+static int fused_threads(const Tiling& T) { return ((T.TY * T.TZ + 63) / 64) * 64; }
+
+template <class Mo>
+static mplbm_status launch_step(mplbm_ctx* ctx, int64_t step, int32_t reverse, uint32_t inject_type, bool macro) {
+  using C = typename Mo::C;
+  const Phys<C> ph = make_phys<C>(ctx, step, reverse, inject_type, macro);
+  const Lattice<Mo> L = make_lattice<Mo>(ctx, ctx->cur, 1 - ctx->cur);
+  if (ctx->kernel == MPLBM_KERNEL_FUSED) {
+    const Tiling& T = ctx->tiling;
+    const int threads = fused_threads(T);
+    const int grid = T.xcd_remap ? ((T.nblocks + 7) / 8) * 8 : T.nblocks;
+    const size_t shm = 4 * (size_t)T.TY * T.TZ * sizeof(C);
+    hipLaunchKernelGGL(k_fused<Mo>, dim3(grid), dim3(threads), shm, ctx->stream, L, ph, T);
+  } else {
+    const int grid = (int)((ctx->ncell + 255) / 256);
+    hipLaunchKernelGGL(k_density<Mo>, dim3(grid), dim3(256), 0, ctx->stream, L, ph);
+    hipLaunchKernelGGL(k_collide<Mo>, dim3(grid), dim3(256), 0, ctx->stream, L, ph);
+  }
+  HIP_TRY(ctx, hipGetLastError());
+  ctx->cur = 1 - ctx->cur;
+  return MPLBM_OK;
+}
+
+template <class F>
+static mplbm_status dispatch(mplbm_ctx* ctx, F&& f) {
+  switch (ctx->cfg.storage) {
+    case MPLBM_FP32: return f(ModeF32{});
+    case MPLBM_FP64: return f(ModeF64{});
+    case MPLBM_FP16S: return f(ModeH16{});
+    case MPLBM_BF16S: return f(ModeB16{});
+  }
+  return fail(ctx, MPLBM_ERR_ARG, "bad storage");
+}
+
+extern "C" {
+
+const char* mplbm_version(void) { return "mplbm 0.1 gfx950"; }
+
+void mplbm_config_default(mplbm_config* c) {
+  std::memset(c, 0, sizeof *c);
+  c->lx = c->ly = c->lz = 400;
+  c->storage = MPLBM_FP32;
+  c->kernel = MPLBM_KERNEL_AUTO;
+  c->tau = 1.0;
+  c->G = -1.0;
+  c->T_Tc = 0.7;
+  c->rhol_spinodal = 0.2726;
+  c->rhog_spinodal = 0.0484;
+  c->rho_inject_period = 100;
+  c->rho_increment = 0.0;
+  c->rhos = 0.0;
+  c->inject_position = 1.0;
+}
+
+const char* mplbm_last_error(const mplbm_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+mplbm_status mplbm_create(const mplbm_config* cfg, mplbm_ctx** out) {
+  if (!cfg || !out) return fail(nullptr, MPLBM_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (cfg->lx < 1 || cfg->ly < 1 || cfg->lz < 1) return fail(nullptr, MPLBM_ERR_ARG, "domain must be positive");
+  const uint64_t n = (uint64_t)cfg->lx * cfg->ly * cfg->lz;
+  if (n >= (1ull << 32)) return fail(nullptr, MPLBM_ERR_ARG, "domain exceeds 2^32 cells");
+  if (cfg->storage < MPLBM_FP32 || cfg->storage > MPLBM_BF16S) return fail(nullptr, MPLBM_ERR_ARG, "bad storage");
+  if (!(cfg->tau > 0.5)) return fail(nullptr, MPLBM_ERR_ARG, "tau must exceed 0.5");
+  if (cfg->carn_star && cfg->G == 0.0) return fail(nullptr, MPLBM_ERR_ARG, "Carnahan-Starling psi needs G != 0");
+  for (int a = 0; a < 3; ++a) {
+    if ((cfg->face_type[2 * a] != 0) != (cfg->face_type[2 * a + 1] != 0))
+      return fail(nullptr, MPLBM_ERR_ARG, "faces of one axis must both be periodic or both non-periodic");
+  }
+  for (int f = 0; f < 6; ++f)
+    if (cfg->face_type[f] < 0 || cfg->face_type[f] > 1) return fail(nullptr, MPLBM_ERR_ARG, "bad face type");
+
+  mplbm_ctx* ctx = new mplbm_ctx();
+  ctx->cfg = *cfg;
+  ctx->ncell = n;
+  ctx->bytes_pop = (int)storage_bytes(cfg->storage);
+  // population arrays start on 256-byte boundaries
+  const uint64_t align_elems = 256 / ctx->bytes_pop;
+  ctx->pstride = (n + align_elems - 1) / align_elems * align_elems;
+  ctx->kernel = cfg->kernel == MPLBM_KERNEL_TWOPASS ? MPLBM_KERNEL_TWOPASS : MPLBM_KERNEL_FUSED;
+  mplbm_status st = choose_tiling(ctx);
+  if (st != MPLBM_OK) {
+    g_create_error = ctx->err;
+    delete ctx;
+    return st;
+  }
+  auto bail = [&](mplbm_status s) {
+    g_create_error = ctx->err;
+    mplbm_destroy(ctx);
+    return s;
+  };
+#define CREATE_TRY(call)                                  \
+  do {                                                    \
+    hipError_t e_ = (call);                               \
+    if (e_ != hipSuccess) {                               \
+      fail(ctx, e_ == hipErrorOutOfMemory ? MPLBM_ERR_NOMEM : MPLBM_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+      return bail(e_ == hipErrorOutOfMemory ? MPLBM_ERR_NOMEM : MPLBM_ERR_HIP); \
+    }                                                     \
+  } while (0)
+  CREATE_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  CREATE_TRY(hipEventCreate(&ctx->ev0));
+  CREATE_TRY(hipEventCreate(&ctx->ev1));
+  const size_t lat_bytes = (size_t)ctx->pstride * kQ * ctx->bytes_pop;
+  const size_t mb = cfg->storage == MPLBM_FP64 ? 8 : 4;  // macro / psi element size
+  CREATE_TRY(hipMalloc(&ctx->lat[0], lat_bytes));
+  CREATE_TRY(hipMalloc(&ctx->lat[1], lat_bytes));
+  CREATE_TRY(hipMalloc((void**)&ctx->link, n * 4));
+  CREATE_TRY(hipMalloc(&ctx->rho, n * mb));
+  CREATE_TRY(hipMalloc(&ctx->pres, n * mb));
+  CREATE_TRY(hipMalloc(&ctx->u, 3 * n * mb));
+  CREATE_TRY(hipMalloc(&ctx->psi, n * mb));
+  ctx->red_blocks = 1024;
+  CREATE_TRY(hipMalloc((void**)&ctx->red_partial, (size_t)ctx->red_blocks * kRedFields * 8));
+  CREATE_TRY(hipMalloc((void**)&ctx->red_out, kRedFields * 8));
+  ctx->device_bytes = (int64_t)(2 * lat_bytes + n * 4 + 6 * n * mb + (size_t)ctx->red_blocks * kRedFields * 8);
+#undef CREATE_TRY
+  *out = ctx;
+  return MPLBM_OK;
+}
+
+void mplbm_destroy(mplbm_ctx* ctx) {
+  if (!ctx) return;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  void* bufs[] = {ctx->lat[0], ctx->lat[1], ctx->link, ctx->mask, ctx->rho, ctx->pres,
+                  ctx->u,      ctx->psi,    ctx->red_partial, ctx->red_out};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+mplbm_status mplbm_set_geometry(mplbm_ctx* ctx, const uint8_t* is_fluid) {
+  if (!ctx || !is_fluid) return fail(ctx, MPLBM_ERR_ARG, "null argument");
+  const uint64_t n = ctx->ncell;
+  if (!ctx->mask) HIP_TRY(ctx, hipMalloc((void**)&ctx->mask, n));
+  std::vector<uint8_t> m(is_fluid, is_fluid + n);
+  int64_t nf = 0;
+  for (uint64_t c = 0; c < n; ++c) {
+    m[c] = m[c] ? 1 : 0;
+    nf += m[c];
+  }
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->mask, m.data(), n, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_links, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, ctx->mask, ctx->link,
+                     (int)ctx->cfg.lx, (int)ctx->cfg.ly, (int)ctx->cfg.lz);
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->fluid_nodes = nf;
+  ctx->have_geometry = true;
+  ctx->have_fields = false;
+  return MPLBM_OK;
+}
+
+// Deviation storage: rho_ref defaults to the mean of the initial fluid
+// density, the scale to a power of two that maps the largest expected
+// deviation (|rho - rho_ref| up to max(rho_ref, 2) ) near 2^13 in FP16;
+// BF16 has the float exponent range and needs no scale.
+static void choose_deviation(mplbm_ctx* ctx, const double* rho) {
+  const mplbm_config& c = ctx->cfg;
+  if (c.storage != MPLBM_FP16S && c.storage != MPLBM_BF16S) {
+    ctx->rho_ref = 0.f;
+    ctx->scale = 1.f;
+    return;
+  }
+  double ref = c.rho_ref;
+  if (ref == 0.0) {
+    double s = 0.0;
+    for (uint64_t k = 0; k < ctx->ncell; ++k) s += rho[k];
+    ref = s / (double)ctx->ncell;
+  }
+  ctx->rho_ref = (float)ref;
+  double sc = c.dev_scale;
+  if (sc == 0.0) {
+    if (c.storage == MPLBM_BF16S) sc = 1.0;
+    else {
+      const double span = std::max(2.0, std::fabs(ref) + 2.0);
+      sc = std::ldexp(1.0, (int)std::floor(std::log2(8192.0 / span)));
+    }
+  }
+  ctx->scale = (float)sc;
+}
+
+mplbm_status mplbm_init_fields_f64(mplbm_ctx* ctx, const double* rho) {
+  if (!ctx || !rho) return fail(ctx, MPLBM_ERR_ARG, "null argument");
+  if (!ctx->have_geometry) return fail(ctx, MPLBM_ERR_STATE, "mplbm_set_geometry must precede mplbm_init_fields");
+  const uint64_t n = ctx->ncell;
+  choose_deviation(ctx, rho);
+  double* d_rho = nullptr;
+  HIP_TRY(ctx, hipMalloc((void**)&d_rho, n * 8));
+  hipError_t e = hipMemcpyAsync(d_rho, rho, n * 8, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    mplbm_status st = dispatch(ctx, [&](auto mo) -> mplbm_status {
+      using Mo = decltype(mo);
+      const auto ph = make_phys<typename Mo::C>(ctx, 0, 1, 0, false);
+      auto L = make_lattice<Mo>(ctx, 0, 0);
+      hipLaunchKernelGGL(k_init<Mo>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, L,
+                         static_cast<typename Mo::S*>(ctx->lat[1]), ph, d_rho);
+      return MPLBM_OK;
+    });
+    (void)st;
+    e = hipGetLastError();
+  }
+  hipError_t e2 = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d_rho);
+  if (e != hipSuccess) return fail(ctx, MPLBM_ERR_HIP, "init: %s", hipGetErrorString(e));
+  if (e2 != hipSuccess) return fail(ctx, MPLBM_ERR_HIP, "init sync: %s", hipGetErrorString(e2));
+  ctx->cur = 0;
+  ctx->have_fields = true;
+  return MPLBM_OK;
+}
+
+mplbm_status mplbm_init_fields(mplbm_ctx* ctx, const float* rho) {
+  if (!ctx || !rho) return fail(ctx, MPLBM_ERR_ARG, "null argument");
+  std::vector<double> d(rho, rho + ctx->ncell);
+  return mplbm_init_fields_f64(ctx, d.data());
+}
+
+mplbm_status mplbm_step(mplbm_ctx* ctx, int64_t step, int32_t reverse, uint32_t inject_type, int32_t nsteps,
+                        uint32_t flags) {
+  if (!ctx) return fail(ctx, MPLBM_ERR_ARG, "null context");
+  if (!ctx->have_fields) return fail(ctx, MPLBM_ERR_STATE, "mplbm_init_fields must precede mplbm_step");
+  if (nsteps < 0) return fail(ctx, MPLBM_ERR_ARG, "nsteps < 0");
+  if (inject_type > 2) return fail(ctx, MPLBM_ERR_ARG, "inject_type must be 0, 1 or 2");
+  for (int32_t s = 0; s < nsteps; ++s) {
+    const bool macro = (flags & MPLBM_STEP_WRITE_MACRO) && s == nsteps - 1;
+    mplbm_status st = dispatch(ctx, [&](auto mo) {
+      return launch_step<decltype(mo)>(ctx, step + s, reverse, inject_type, macro);
+    });
+    if (st != MPLBM_OK) return st;
+  }
+  return MPLBM_OK;
+}
+
+mplbm_status mplbm_step_timed(mplbm_ctx* ctx, int64_t step, int32_t reverse, uint32_t inject_type,
+                              int32_t nsteps, uint32_t flags, double* elapsed_ms) {
+  if (!ctx) return fail(ctx, MPLBM_ERR_ARG, "null context");
+  HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  mplbm_status st = mplbm_step(ctx, step, reverse, inject_type, nsteps, flags);
+  if (st != MPLBM_OK) return st;
+  HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+  float ms = 0.f;
+  HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  if (elapsed_ms) *elapsed_ms = ms;
+  return MPLBM_OK;
+}
+
+mplbm_status mplbm_sync(mplbm_ctx* ctx) {
+  if (!ctx) return fail(ctx, MPLBM_ERR_ARG, "null context");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return MPLBM_OK;
+}
+
+}  // extern "C"
+
+template <class T>
+static mplbm_status read_fields_t(mplbm_ctx* ctx, T* rho, T* pres, T* u) {
+  if (!ctx) return fail(ctx, MPLBM_ERR_ARG, "null context");
+  if (!ctx->have_fields) return fail(ctx, MPLBM_ERR_STATE, "fields not initialised");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  const uint64_t n = ctx->ncell;
+  const bool f64 = ctx->cfg.storage == MPLBM_FP64;
+  const size_t eb = f64 ? 8 : 4;
+  std::vector<unsigned char> buf(3 * n * eb);
+  std::vector<uint32_t> lk(n);
+  HIP_TRY(ctx, hipMemcpy(lk.data(), ctx->link, n * 4, hipMemcpyDeviceToHost));
+  auto conv = [&](const void* dev, T* dst, uint64_t count, bool is_u) -> mplbm_status {
+    HIP_TRY(ctx, hipMemcpy(buf.data(), dev, count * eb, hipMemcpyDeviceToHost));
+    for (uint64_t k = 0; k < count; ++k) {
+      const uint64_t cell = is_u ? k % n : k;
+      double v = f64 ? reinterpret_cast<const double*>(buf.data())[k] : reinterpret_cast<const float*>(buf.data())[k];
+      (void)cell;
+      if (is_u) {
+        // device u is [3][cell]; host wants [cell][3]
+        const uint64_t comp = k / n;
+        dst[cell * 3 + comp] = (lk[cell] & 1u) ? (T)0 : (T)v;
+      } else {
+        dst[k] = (T)v;
+      }
+    }
+    return MPLBM_OK;
+  };
+  mplbm_status st = MPLBM_OK;
+  if (rho && (st = conv(ctx->rho, rho, n, false)) != MPLBM_OK) return st;
+  if (pres && (st = conv(ctx->pres, pres, n, false)) != MPLBM_OK) return st;
+  if (pres)
+    for (uint64_t k = 0; k < n; ++k)
+      if (lk[k] & 1u) pres[k] = (T)0;
+  if (u && (st = conv(ctx->u, u, 3 * n, true)) != MPLBM_OK) return st;
+  return MPLBM_OK;
+}
+
+extern "C" {
+
+mplbm_status mplbm_read_fields(mplbm_ctx* ctx, float* rho, float* pressure, float* u) {
+  return read_fields_t<float>(ctx, rho, pressure, u);
+}
+mplbm_status mplbm_read_fields_f64(mplbm_ctx* ctx, double* rho, double* pressure, double* u) {
+  return read_fields_t<double>(ctx, rho, pressure, u);
+}
+
+mplbm_status mplbm_read_populations_f64(mplbm_ctx* ctx, double* f) {
+  if (!ctx || !f) return fail(ctx, MPLBM_ERR_ARG, "null argument");
+  if (!ctx->have_fields) return fail(ctx, MPLBM_ERR_STATE, "fields not initialised");
+  const uint64_t n = ctx->ncell;
+  double* d = nullptr;
+  HIP_TRY(ctx, hipMalloc((void**)&d, n * kQ * 8));
+  mplbm_status st = dispatch(ctx, [&](auto mo) -> mplbm_status {
+    using Mo = decltype(mo);
+    const auto ph = make_phys<typename Mo::C>(ctx, 0, 1, 0, false);
+    auto L = make_lattice<Mo>(ctx, ctx->cur, ctx->cur);
+    hipLaunchKernelGGL(k_read_pops<Mo>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, L, ph, d);
+    return MPLBM_OK;
+  });
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(f, d, n * kQ * 8, hipMemcpyDeviceToHost, ctx->stream);
+  hipError_t e2 = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d);
+  if (st != MPLBM_OK) return st;
+  if (e != hipSuccess || e2 != hipSuccess)
+    return fail(ctx, MPLBM_ERR_HIP, "read populations: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+  return MPLBM_OK;
+}
+
+mplbm_status mplbm_write_populations_f64(mplbm_ctx* ctx, const double* f) {
+  if (!ctx || !f) return fail(ctx, MPLBM_ERR_ARG, "null argument");
+  if (!ctx->have_fields) return fail(ctx, MPLBM_ERR_STATE, "fields not initialised");
+  const uint64_t n = ctx->ncell;
+  double* d = nullptr;
+  HIP_TRY(ctx, hipMalloc((void**)&d, n * kQ * 8));
+  hipError_t e = hipMemcpyAsync(d, f, n * kQ * 8, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    dispatch(ctx, [&](auto mo) -> mplbm_status {
+      using Mo = decltype(mo);
+      const auto ph = make_phys<typename Mo::C>(ctx, 0, 1, 0, false);
+      auto L = make_lattice<Mo>(ctx, ctx->cur, ctx->cur);
+      hipLaunchKernelGGL(k_write_pops<Mo>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, L, ph, d);
+      return MPLBM_OK;
+    });
+    e = hipGetLastError();
+  }
+  hipError_t e2 = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess || e2 != hipSuccess)
+    return fail(ctx, MPLBM_ERR_HIP, "write populations: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+  return MPLBM_OK;
+}
+
+mplbm_status mplbm_reduce(mplbm_ctx* ctx, mplbm_stats64* out) {
+  if (!ctx || !out) return fail(ctx, MPLBM_ERR_ARG, "null argument");
+  if (!ctx->have_fields) return fail(ctx, MPLBM_ERR_STATE, "fields not initialised");
+  mplbm_status st = dispatch(ctx, [&](auto mo) -> mplbm_status {
+    using Mo = decltype(mo);
+    const auto ph = make_phys<typename Mo::C>(ctx, 0, 1, 0, false);
+    auto L = make_lattice<Mo>(ctx, ctx->cur, ctx->cur);
+    hipLaunchKernelGGL(k_reduce<Mo>, dim3(ctx->red_blocks), dim3(kRedThreads), 0, ctx->stream, L, ph,
+                       ctx->cfg.rhol_spinodal, ctx->cfg.rhog_spinodal, ctx->red_partial);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, ctx->stream, ctx->red_partial, ctx->red_blocks,
+                       ctx->red_out);
+    return MPLBM_OK;
+  });
+  if (st != MPLBM_OK) return st;
+  HIP_TRY(ctx, hipGetLastError());
+  double h[kRedFields];
+  HIP_TRY(ctx, hipMemcpyAsync(h, ctx->red_out, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  out->mass = h[0];
+  out->momentum[0] = h[1];
+  out->momentum[1] = h[2];
+  out->momentum[2] = h[3];
+  out->fluid_nodes = h[4];
+  out->liquid_nodes = h[5];
+  out->vapor_nodes = h[6];
+  out->liquid_rho_sum = h[7];
+  out->vapor_rho_sum = h[8];
+  out->liquid_p_sum = h[9];
+  out->vapor_p_sum = h[10];
+  out->rho_min = h[11];
+  out->rho_max = h[12];
+  out->p_min = h[13];
+  out->p_max = h[14];
+  out->nonfinite = h[15];
+  return MPLBM_OK;
+}
+
+mplbm_status mplbm_info_get(const mplbm_ctx* ctx, mplbm_info* out) {
+  if (!ctx || !out) return MPLBM_ERR_ARG;
+  out->fluid_nodes = ctx->fluid_nodes;
+  out->cells = (int64_t)ctx->ncell;
+  out->storage = ctx->cfg.storage;
+  out->kernel = ctx->kernel;
+  out->tile_y = ctx->tiling.TY;
+  out->tile_z = ctx->tiling.TZ;
+  out->seg_x = ctx->tiling.seg;
+  out->bytes_per_pop = ctx->bytes_pop;
+  out->rho_ref = ctx->rho_ref;
+  out->dev_scale = ctx->scale;
+  out->device_bytes = ctx->device_bytes;
+  return MPLBM_OK;
+}
+
+}  // extern "C"
