@@ -1,0 +1,204 @@
+#!/usr/bin/env python3
+"""Throughput of the D3Q19 Shan-Chen step on one MI355X.
+
+Workload (the packing the headline metric is quoted on): 512^3 domain,
+overlapping spheres with radii uniform in [6, 18] placed by a seeded
+generator until the voxel porosity is 0.45, Shan-Chen G = -5, tau = 1,
+rho0 = 0.7 +/- 0.01 (srand(1) initialisation of the reference).  A step is
+one fused stream-collide launch over the whole lattice.
+
+Prints one JSON line: value = fluid-node MLUPS of the FP32 form (whole job,
+inputs resident in HBM), the roofline object of the fused kernel (algorithmic
+bytes 2*19*4 = 152 per fluid update / HIP-event-timed average launch
+duration on the context stream), the FP16-storage form beside it, and the
+CPU baseline (the oracle, OpenMP, on a bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+Q = 19
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--porosity", type=float, default=0.45)
+    ap.add_argument("--rmin", type=float, default=6.0)
+    ap.add_argument("--rmax", type=float, default=18.0)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--storage", default="fp32,fp16s,bf16s")
+    ap.add_argument("--kernel", default="fused")
+    ap.add_argument("--tile-y", type=int, default=0)
+    ap.add_argument("--tile-z", type=int, default=0)
+    ap.add_argument("--seg-x", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline (0: skip)")
+    ap.add_argument("--profile-only", action="store_true", help="run only the FP32 timed loop (for rocprofv3)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a plain device-to-device copy kernel of 2 GiB (rocprofv3 --pmc calibration)")
+    return ap.parse_args()
+
+
+def workload(a):
+    from multiphase_lbm_amd import harness as H
+    shape = (a.size, a.size, a.size)
+    t = time.time()
+    fluid, sph = H.generate_packing(shape, a.seed, 10 ** 8, a.rmin, a.rmax, a.porosity, nonoverlap=False)
+    rho = H.rho_init(shape, 0.7, 0.02 / 0.7)
+    print(f"# workload {shape}: {len(sph)} spheres, porosity {fluid.mean():.4f}, built in {time.time() - t:.1f}s",
+          file=sys.stderr, flush=True)
+    return shape, fluid, rho, len(sph)
+
+
+PHYS = dict(tau=1.0, G=-5.0, rhos=0.7, rhol_spinodal=1.285931, rhog_spinodal=0.323507)
+
+
+def run_gpu(a, shape, fluid, rho, storage):
+    import multiphase_lbm_amd as mp
+    ctx = mp.Context(*shape, storage=storage, kernel=a.kernel, tile_y=a.tile_y, tile_z=a.tile_z, seg_x=a.seg_x,
+                     **PHYS)
+    try:
+        ctx.set_geometry(fluid)
+        ctx.init_fields(rho)
+        info = ctx.info()
+        ctx.step(0, a.warmup)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ev_ms = ctx.step_timed(a.warmup, a.steps)  # HIP events on the context stream around the K launches
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        stats = ctx.reduce()
+    finally:
+        ctx.close()
+    nf = info["fluid_nodes"]
+    launches = a.steps * (1 if a.kernel == "fused" else 2)
+    per_launch_s = ev_ms / 1e3 / launches
+    bpop = mp.STORAGE_BYTES[mp.STORAGE_NAMES[storage]]
+    alg_bytes = 2 * Q * bpop * nf  # per step
+    return {
+        "mlups": nf * a.steps / wall / 1e6,
+        "mlups_event": nf * a.steps / (ev_ms / 1e3) / 1e6,
+        "ms_per_step": wall * 1e3 / a.steps,
+        "kernel_ms": per_launch_s * 1e3,
+        "achieved_gbs": alg_bytes / (ev_ms / 1e3 / a.steps) / 1e9,
+        "bytes_per_update": 2 * Q * bpop,
+        "fluid_nodes": nf,
+        "tile": [info["tile_y"], info["tile_z"], info["seg_x"]],
+        "mass": stats["mass"],
+        "nonfinite": stats["nonfinite"],
+    }
+
+
+def cpu_baseline(a, shape, fluid, rho):
+    """The oracle (C, OpenMP, reference float arithmetic) on the same 512^3
+    workload for a bounded number of steps."""
+    from oracle import oracle
+    P = oracle.Params(*shape, **PHYS)
+    st = oracle.State(P, fluid, rho, dtype=np.float32)
+    t = time.perf_counter()
+    st.run(a.cpu_steps)
+    dt = time.perf_counter() - t
+    nf = int(fluid.sum())
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    del st
+    return {"value": nf * a.cpu_steps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"oracle (float, 3-sweep AoS restatement of the reference kernels) {a.cpu_steps} steps "
+                      f"of the same {shape[0]}^3 packing, {dt:.1f}s"}
+
+
+def traffic_from_profiles(storage):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration) for this workload, as
+    committed in profiles/traffic_fused_512.json; None when absent."""
+    path = os.path.join(ROOT, "profiles", "traffic_fused_512.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(storage)
+    except (OSError, ValueError):
+        return None
+
+
+def copy_calibration():
+    """A plain copy kernel of known size: FETCH_SIZE/WRITE_SIZE of this run
+    against the 2 GiB read + 2 GiB written give the counter scale."""
+    import torch
+    n = 1 << 29  # 2 GiB of float32
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}), flush=True)
+
+
+def main():
+    a = parse()
+    if a.copy_calibration:
+        copy_calibration()
+        return
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU slab decomposition is "
+              "a design note, not built", file=sys.stderr)
+        sys.exit(2)
+    shape, fluid, rho, nsph = workload(a)
+    storages = ["fp32"] if a.profile_only else a.storage.split(",")
+    res = {}
+    for s in storages:
+        res[s] = run_gpu(a, shape, fluid, rho, s)
+        print(f"# {s}: {json.dumps(res[s])}", file=sys.stderr, flush=True)
+    if a.profile_only:
+        return
+    cpu = cpu_baseline(a, shape, fluid, rho) if a.cpu_steps > 0 else None
+    r = res["fp32"]
+    line = {
+        "metric": "MLUPS + % HBM roofline, 512^3 packing: FP32 vs FP16-storage at stated error",
+        "value": round(r["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(r["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"D3Q19 Shan-Chen step, {shape[0]}x{shape[1]}x{shape[2]} sphere packing "
+                               f"(porosity {fluid.mean():.3f}, {nsph} overlapping spheres r~U[{a.rmin},{a.rmax}], "
+                               f"seed {a.seed}), G=-5, tau=1, rho0=0.7+/-0.01, fused {a.kernel} kernel",
+                   "fluid_nodes": r["fluid_nodes"], "tile_y_z_seg": r["tile"],
+                   "mlups_unit": "fluid-node updates per second / 1e6"},
+        "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles("fp32"),
+                     "bytes_per_update": r["bytes_per_update"], "kernel_ms": round(r["kernel_ms"], 4)},
+        "cpu_baseline": cpu,
+    }
+    for s in storages:
+        if s == "fp32":
+            continue
+        q = res[s]
+        line[s] = {"value": round(q["mlups"], 1), "unit": "MLUPS", "vs_fp32": round(q["mlups"] / r["mlups"], 3),
+                   "roofline": {"bound": "hbm", "achieved": round(q["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
+                                "unit": "GB/s", "frac": round(q["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                                "traffic": traffic_from_profiles(s), "bytes_per_update": q["bytes_per_update"],
+                                "kernel_ms": round(q["kernel_ms"], 4)}}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,133 @@
+/*
+ * mplbm_host — C ABI of the host harness around libmplbm (libmplbm_host.so):
+ * the reference's input-file parser, sphere voxeliser, positions and sample
+ * loaders, density initialisation, statistics and VTK/Stats writers, and the
+ * driver loop, re-implemented in C++17 with runtime domain sizes.
+ *
+ * Field arrays are [x][y][z] (z fastest) as in the reference
+ * (include/lbm_model.cuh:18-22).  Functions return 0 on success and a
+ * negative value on error; mplbm_host_last_error() gives the text.
+ */
+#ifndef MPLBM_HOST_H_
+#define MPLBM_HOST_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Parsed input file (tokens of src/multiphase_lbm.cu:631-725 plus the
+ * runtime-size and boundary tokens of this build). */
+typedef struct mplbm_input {
+  int64_t lx, ly, lz;
+  char result_folder[256];
+  int32_t num_threads;
+  int64_t step, step_max;
+  int64_t vtkPeriod, vtrPeriod, tecplotPeriod, statPeriod;
+  int32_t save_vtk_ascii, save_vtr_binary, save_tecplot_ascii;
+  double rho0, rhos, G, IniPerturbRate, tau;
+  double res;
+  int64_t rho_inject_period;
+  double rho_increment;
+  int32_t inject_density;
+  uint32_t inject_type;
+  double inject_position;
+  int32_t carn_star;
+  double T_Tc, rhol_spinodal, rhog_spinodal;
+  int32_t read_densities;
+  char densities_filename[256];
+  int32_t hysteresis;
+  double max_saturation, min_saturation;
+  int32_t erosion;
+  double target_density;
+  /* build extensions */
+  int32_t storage;          /* mplbm_storage */
+  int32_t kernel;           /* mplbm_kernel */
+  int32_t face_type[6];
+  double face_rho[6];
+  double body_force[3];
+  int32_t unknown_tokens;   /* tokens reported as undefined */
+} mplbm_input;
+
+/* Statistics of MultiphaseLBM::calc_cluster (src/multiphase_lbm.cu:225-562). */
+typedef struct mplbm_cluster_stats {
+  double min_rho, max_rho, min_p, max_p;
+  double avg_liq_press, avg_vap_press, saturation, avg_liq_rho, avg_vap_rho;
+  int64_t liq_cluster_count, gas_cluster_count;
+  double cluster_order, wet_coord_num;
+  double ratio_liquid_solid_boundary, ratio_liquid_vapor_boundary;
+  double cluster_radius;
+  int64_t fluid_count, liquid_count, vapor_count;
+} mplbm_cluster_stats;
+
+const char* mplbm_host_last_error(void);
+
+/* Input file defaults (src/multiphase_lbm.cu:35-80, domain 400^3). */
+void mplbm_host_input_default(mplbm_input* in);
+
+/* Token parser only (no geometry work): fills `in` from the file.  Tokens
+ * that act on the domain (resize, read_sample, place_sphere,
+ * generate_solids, read_positions) are counted but not executed. */
+int mplbm_host_parse(const char* path, mplbm_input* in);
+
+/* place_sphere (src/multiphase_lbm.cu:893-927): float stepping, sample-point
+ * test dist2 < R^2, nearest() rounding, one periodic wrap.  Marks is_fluid
+ * = 0 and solid_number = solid_id (solid_number may be NULL). */
+void mplbm_host_place_sphere(uint8_t* is_fluid, int64_t* solid_number, int64_t lx, int64_t ly, int64_t lz,
+                             float x, float y, float z, float R, int64_t solid_id);
+
+/* read_positions (src/multiphase_lbm.cu:1106-1126): x,y,z,r one per line,
+ * each multiplied by res.  Returns the number of spheres placed or < 0. */
+int64_t mplbm_host_read_positions(const char* path, float res, uint8_t* is_fluid, int64_t* solid_number,
+                                  int64_t lx, int64_t ly, int64_t lz, int64_t first_id);
+
+/* Density initialisation of resize_gpu (src/multiphase_lbm.cu:134-144):
+ * srand(1), loops z, y, x, rho0*(1 + IniPerturbRate*(rand()/RAND_MAX - 0.5)). */
+void mplbm_host_rho_init(float* rho, int64_t lx, int64_t ly, int64_t lz, float rho0, float ini_perturb_rate);
+
+/* read_densities (src/multiphase_lbm.cu:112-132): Density block of a
+ * legacy VTK written by mplbm_host_write_vtk.  Returns values read or < 0. */
+int64_t mplbm_host_read_densities(const char* path, float* rho, int64_t lx, int64_t ly, int64_t lz);
+
+/* read_sample (src/multiphase_lbm.cu:750-806): 0/1 voxel file with a
+ * 3-number header, down-sampled by lx_file/lx.  Returns 0 or < 0. */
+int mplbm_host_read_sample(const char* path, uint8_t* is_fluid, int64_t lx, int64_t ly, int64_t lz);
+
+/* Seeded synthetic packing (not in the reference): spheres with radii
+ * uniform in [rmin, rmax] and centres uniform in the box, placed until
+ * n_max spheres or the voxel porosity target is reached.  nonoverlap = 1
+ * rejects spheres overlapping earlier ones (random sequential addition,
+ * periodic minimum-image distance).  Spheres are voxelised with
+ * mplbm_host_place_sphere.  spheres_out receives x,y,z,r (capacity
+ * spheres).  Returns the number of spheres placed or < 0. */
+int64_t mplbm_host_generate_packing(uint8_t* is_fluid, int64_t lx, int64_t ly, int64_t lz, uint64_t seed,
+                                    int64_t n_max, double rmin, double rmax, double porosity_target,
+                                    int32_t nonoverlap, double* spheres_out, int64_t capacity);
+
+/* Write spheres in the positions-file format read by read_positions. */
+int mplbm_host_write_positions(const char* path, const double* spheres, int64_t n);
+
+/* calc_cluster + write_stats extrema (src/multiphase_lbm.cu:225-562,
+ * 1057-1083).  liq_label/gas_label (may be NULL) receive the cluster numbers
+ * written to the VTK LiqClusters/GasClusters blocks. */
+int mplbm_host_cluster_stats(const uint8_t* is_fluid, const float* rho, const float* pres,
+                             const int64_t* solid_number, int64_t lx, int64_t ly, int64_t lz,
+                             double rhol_spinodal, double rhog_spinodal, int64_t solid_count,
+                             int64_t* liq_label, int64_t* gas_label, mplbm_cluster_stats* out);
+
+/* write_vtk_ascii (src/multiphase_lbm.cu:948-1054), same format strings. */
+int mplbm_host_write_vtk(const char* path, const uint8_t* is_fluid, const float* rho, const float* pres,
+                         const int64_t* liq_label, const int64_t* gas_label, const int64_t* solid_number,
+                         int64_t lx, int64_t ly, int64_t lz);
+
+/* The whole program of src/main.cc:4-17 (read_data, init, run_gpu) on the
+ * HIP library.  Returns 0 on success. */
+int mplbm_host_run(const char* input_path);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPLBM_HOST_H_ */

@@ -56,7 +56,8 @@ def build_host() -> str:
     srcs = [os.path.join(CSRC, "host", f) for f in ("harness.cpp",)]
     tmp = HOST_LIB + ".tmp"
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wall",
-          "-shared", f"-I{INCLUDE}", "-o", tmp, *srcs])
+          "-shared", f"-I{INCLUDE}", "-o", tmp, *srcs, f"-L{PKG}", "-lmplbm", "-Wl,-rpath,$ORIGIN",
+          "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, HOST_LIB)
     return HOST_LIB
 

@@ -28,7 +28,7 @@ def _case(shape, nsph, rmin, rmax, seed, rho0=0.7, amp=0.01):
 def _run_pair(shape, fluid, rho, steps, storage="fp32", kernel="fused", step0=0, reverse=1, inject_type=0,
               **params):
     dtype = np.float64 if storage == "fp64" else np.float32
-    P = oracle.Params(*shape, **params)
+    P = oracle.Params(*shape, **{k: v for k, v in params.items() if k not in LIB_ONLY})
     st = oracle.State(P, fluid, rho.astype(dtype), dtype=dtype)
     st.step = step0
     st.run(steps, reverse=reverse, inject_type=inject_type)
@@ -42,9 +42,11 @@ def _run_pair(shape, fluid, rho, steps, storage="fp32", kernel="fused", step0=0,
     return st, (g_rho, g_p, g_u, g_f, stats)
 
 
+LIB_ONLY = ("tile_y", "tile_z", "seg_x", "rho_ref", "dev_scale")
+
+
 def _lib_params(params):
-    out = dict(params)
-    return out
+    return dict(params)
 
 
 def _relerr(a, b):
