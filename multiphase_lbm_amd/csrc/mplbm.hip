@@ -1285,3 +1285,100 @@ mplbm_status mplbm_info_get(const mplbm_ctx* ctx, mplbm_info* out) {
 }
 
 }  // extern "C"
+
+// ============================================================================
+// reference-named entry points (include/iteration_host_compat.h)
+
+#include "iteration_host_compat.h"
+
+namespace {
+struct Compat {
+  mplbm_config cfg{};
+  bool have_domain = false, have_params = false;
+  mplbm_ctx* ctx = nullptr;
+  float* rho = nullptr;
+  float* pres = nullptr;
+};
+Compat g_compat;
+}  // namespace
+
+extern "C" {
+
+int mplbm_compat_set_domain(int64_t lx, int64_t ly, int64_t lz, int32_t storage) {
+  if (!g_compat.have_params) mplbm_config_default(&g_compat.cfg);
+  g_compat.cfg.lx = lx;
+  g_compat.cfg.ly = ly;
+  g_compat.cfg.lz = lz;
+  g_compat.cfg.storage = storage;
+  g_compat.have_domain = true;
+  return MPLBM_OK;
+}
+
+int transfer_parameter(const float* tau, const float* G, const bool* carn_star, const float* T_Tc,
+                       const float* rhol_spinodal, const float* rhog_spinodal, const bool* inject_density,
+                       const size_t* rho_inject_period, const float* rho_increment, const float* rhos,
+                       const float* inject_position, const size_t* statPeriod) {
+  if (!tau || !G || !carn_star || !T_Tc || !rhol_spinodal || !rhog_spinodal || !inject_density ||
+      !rho_inject_period || !rho_increment || !rhos || !inject_position || !statPeriod)
+    return fail(nullptr, MPLBM_ERR_ARG, "transfer_parameter: null argument");
+  if (!g_compat.have_domain) {
+    mplbm_config_default(&g_compat.cfg);
+    g_compat.have_domain = true;
+  }
+  mplbm_config& c = g_compat.cfg;
+  c.tau = *tau;
+  c.G = *G;
+  c.carn_star = *carn_star ? 1 : 0;
+  c.T_Tc = *T_Tc;
+  c.rhol_spinodal = *rhol_spinodal;
+  c.rhog_spinodal = *rhog_spinodal;
+  // the reference copies inject_density but never reads it on the device
+  (void)inject_density;
+  c.rho_inject_period = (int64_t)*rho_inject_period;
+  c.rho_increment = *rho_increment;
+  c.rhos = *rhos;
+  c.inject_position = *inject_position;
+  (void)statPeriod;  // copied to the device but unused there (iteration_host.cu:39)
+  g_compat.have_params = true;
+  return MPLBM_OK;
+}
+
+int multiphaselbm_initialization(float* rho, float* pressure, const bool* is_fluid) {
+  if (!rho || !pressure || !is_fluid) return fail(nullptr, MPLBM_ERR_ARG, "multiphaselbm_initialization: null");
+  if (!g_compat.have_params) return fail(nullptr, MPLBM_ERR_STATE, "transfer_parameter must come first");
+  mplbm_compat_release();
+  mplbm_ctx* ctx = nullptr;
+  mplbm_status st = mplbm_create(&g_compat.cfg, &ctx);
+  if (st != MPLBM_OK) return st;
+  const uint64_t n = ctx->ncell;
+  std::vector<uint8_t> m(n);
+  for (uint64_t k = 0; k < n; ++k) m[k] = is_fluid[k] ? 1 : 0;
+  if ((st = mplbm_set_geometry(ctx, m.data())) != MPLBM_OK || (st = mplbm_init_fields(ctx, rho)) != MPLBM_OK ||
+      (st = mplbm_read_fields(ctx, rho, pressure, nullptr)) != MPLBM_OK) {
+    g_create_error = ctx->err;
+    mplbm_destroy(ctx);
+    return st;
+  }
+  g_compat.ctx = ctx;
+  g_compat.rho = rho;
+  g_compat.pres = pressure;
+  return MPLBM_OK;
+}
+
+int multiphaselbm_iteration(long step, size_t statPeriod, int reverse, unsigned int inject_type) {
+  mplbm_ctx* ctx = g_compat.ctx;
+  if (!ctx) return fail(nullptr, MPLBM_ERR_STATE, "multiphaselbm_initialization must come first");
+  const bool stat = statPeriod > 0 && (uint64_t)step % statPeriod == 0;
+  mplbm_status st = mplbm_step(ctx, step, reverse, inject_type, 1, stat ? MPLBM_STEP_WRITE_MACRO : 0);
+  if (st != MPLBM_OK) return st;
+  if (stat) return mplbm_read_fields(ctx, g_compat.rho, g_compat.pres, nullptr);
+  return MPLBM_OK;
+}
+
+void mplbm_compat_release(void) {
+  if (g_compat.ctx) mplbm_destroy(g_compat.ctx);
+  g_compat.ctx = nullptr;
+  g_compat.rho = g_compat.pres = nullptr;
+}
+
+}  // extern "C"

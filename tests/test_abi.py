@@ -1,0 +1,66 @@
+"""The C-ABI libraries load on a machine without a GPU and export every
+function their public headers declare (no compute calls here)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(mplbm_\w+|transfer_parameter|multiphaselbm_\w+)\s*\(", text)
+    return sorted(set(n for n in names if not n.endswith("_t")))
+
+
+def _exported(lib):
+    out = subprocess.run(["nm", "-D", "--defined-only", lib], capture_output=True, text=True, check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if " T " in line}
+
+
+@pytest.mark.parametrize("header,lib", [("mplbm.h", "libmplbm.so"), ("mplbm_host.h", "libmplbm_host.so"),
+                                        ("iteration_host_compat.h", "libmplbm.so")])
+def test_library_exports_every_declared_symbol(header, lib):
+    path = os.path.join(ROOT, "multiphase_lbm_amd", lib)
+    assert os.path.exists(path), f"{lib} not built"
+    declared = _declared(header)
+    assert declared, header
+    missing = [n for n in declared if n not in _exported(path)]
+    assert not missing, f"{lib} lacks {missing}"
+    L = ctypes.CDLL(path)
+    for n in declared:
+        getattr(L, n)
+
+
+def test_python_binding_covers_the_abi():
+    from multiphase_lbm_amd import _abi, harness
+    assert set(_abi.SIGNATURES) == set(_declared("mplbm.h"))
+    assert set(harness.SIGNATURES) == set(_declared("mplbm_host.h"))
+
+
+def test_config_layout_matches_c():
+    """The ctypes mirror of mplbm_config has the C struct's size (checked
+    against the library's default-filled struct)."""
+    from multiphase_lbm_amd import _abi
+    assert ctypes.sizeof(_abi.Config) == 240
+    cfg = _abi.Config()
+    _abi.load().mplbm_config_default(ctypes.byref(cfg))
+    assert (cfg.lx, cfg.ly, cfg.lz) == (400, 400, 400)
+    assert cfg.tau == 1.0 and cfg.G == -1.0 and cfg.rho_inject_period == 100
+    assert abs(cfg.rhol_spinodal - 0.2726) < 1e-15 and cfg.inject_position == 1.0
+
+
+def test_create_reports_errors_without_exit():
+    """Invalid configurations return a status and a message (the reference
+    exits the process, include/iteration_host.h:6-15)."""
+    import multiphase_lbm_amd as mp
+    with pytest.raises(mp.MplbmError, match="tau"):
+        mp.Context(8, 8, 8, tau=0.5)
+    with pytest.raises(mp.MplbmError, match="faces"):
+        mp.Context(8, 8, 8, face_type=(1, 0, 0, 0, 0, 0))
+    with pytest.raises(mp.MplbmError, match="2\\^32"):
+        mp.Context(2048, 2048, 2048)
