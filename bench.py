@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--seg-x", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline (0: skip)")
     ap.add_argument("--profile-only", action="store_true", help="run only the FP32 timed loop (for rocprofv3)")
+    ap.add_argument("--sweep", action="store_true", help="time every autotuner tile candidate (FP32) and exit")
+    ap.add_argument("--autotune", action="store_true", help="pick the tile with the autotuner before timing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain device-to-device copy kernel of 2 GiB (rocprofv3 --pmc calibration)")
     return ap.parse_args()
@@ -74,6 +76,11 @@ def run_gpu(a, shape, fluid, rho, storage):
     try:
         ctx.set_geometry(fluid)
         ctx.init_fields(rho)
+        if a.autotune:
+            from multiphase_lbm_amd.autotune import autotune
+            best, _ = autotune(ctx, steps=5, rho=rho)
+            print(f"# autotune {storage}: tile y,z,seg = {best[0]} ({best[1]:.3f} ms/step)", file=sys.stderr,
+                  flush=True)
         info = ctx.info()
         ctx.step(0, a.warmup)
         ctx.sync()
@@ -155,6 +162,19 @@ def main():
               "a design note, not built", file=sys.stderr)
         sys.exit(2)
     shape, fluid, rho, nsph = workload(a)
+    if a.sweep:
+        import multiphase_lbm_amd as mp
+        from multiphase_lbm_amd.autotune import autotune
+        for storage in a.storage.split(","):
+            with mp.Context(*shape, storage=storage, kernel="fused", **PHYS) as ctx:
+                ctx.set_geometry(fluid)
+                ctx.init_fields(rho)
+                nf = ctx.info()["fluid_nodes"]
+                best, res = autotune(ctx, steps=5, verbose=False)
+                for (ty, tz, seg), ms in sorted(res, key=lambda r: r[1]):
+                    print(json.dumps({"storage": storage, "tile_y": ty, "tile_z": tz, "seg_x": seg,
+                                      "ms": round(ms, 4), "mlups": round(nf / ms / 1e3, 1)}), flush=True)
+        return
     storages = ["fp32"] if a.profile_only else a.storage.split(",")
     res = {}
     for s in storages:

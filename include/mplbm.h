@@ -191,6 +191,11 @@ mplbm_status mplbm_write_populations_f64(mplbm_ctx* ctx, const double* f);
  * (synchronises). */
 mplbm_status mplbm_reduce(mplbm_ctx* ctx, mplbm_stats64* out);
 
+/* Changes the FUSED kernel's tile (threads in y and z, planes per block);
+ * 0 keeps the tuned default for that field.  Used by the autotuner
+ * (multiphase_lbm_amd/autotune.py); results do not depend on the tile. */
+mplbm_status mplbm_set_tiling(mplbm_ctx* ctx, int32_t tile_y, int32_t tile_z, int32_t seg_x);
+
 /* Context facts (no synchronisation). */
 mplbm_status mplbm_info_get(const mplbm_ctx* ctx, mplbm_info* out);
 

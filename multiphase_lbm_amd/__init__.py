@@ -149,6 +149,9 @@ class Context:
         self._check(self.lib.mplbm_reduce(self.h, ctypes.byref(s)))
         return s.as_dict()
 
+    def set_tiling(self, tile_y=0, tile_z=0, seg_x=0):
+        self._check(self.lib.mplbm_set_tiling(self.h, int(tile_y), int(tile_z), int(seg_x)))
+
     def info(self) -> dict:
         i = _abi.Info()
         self._check(self.lib.mplbm_info_get(self.h, ctypes.byref(i)))

@@ -79,6 +79,7 @@ SIGNATURES = {
     "mplbm_read_populations_f64": (ctypes.c_int, [VP, VP]),
     "mplbm_write_populations_f64": (ctypes.c_int, [VP, VP]),
     "mplbm_reduce": (ctypes.c_int, [VP, ctypes.POINTER(Stats64)]),
+    "mplbm_set_tiling": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "mplbm_info_get": (ctypes.c_int, [VP, ctypes.POINTER(Info)]),
     "mplbm_version": (ctypes.c_char_p, []),
 }

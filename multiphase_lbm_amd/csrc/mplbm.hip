@@ -1,9 +1,11 @@
 // libmplbm: MI355X (gfx950) D3Q19 Shan-Chen multiphase LBM step behind the
 // C ABI of include/mplbm.h.
 //
-// Device layout (structure of arrays): population i of cell c lives at
-// lattice[i * pstride + c], c = (x*ly + y)*lz + z, so a wavefront reading 64
-// consecutive z of one direction issues one coalesced 256-byte access.  Two
+// Device layout (plane-major structure of arrays): population i of the cell
+// (x, y, z) lives at lattice[(x*19 + i)*ly*lz + y*lz + z].  A wavefront
+// reading 64 consecutive z of one direction issues one coalesced 256-byte
+// access, and the 19 sub-planes of one x-plane lie within a 32-bit byte
+// offset of a single wave-uniform base, which the marching kernel exploits.  Two
 // lattices (pull scheme, A -> B, pointer swap on the host instead of the
 // reference's do_swap kernel, include/iteration_kernel.cuh:44-48).  A 32-bit
 // link word per cell holds bit 0 = "cell is solid" and bit i = "upstream
@@ -326,7 +328,11 @@ struct Lattice {
   typename Mo::M* pres;
   typename Mo::M* u;     // 3 planes of ncell
   typename Mo::C* psi;   // two-pass psi field
-  uint64_t pstride;      // elements between population arrays
+  // element index of population i of cell c (plane-major layout)
+  __device__ __forceinline__ uint64_t pix(int i, uint32_t c) const {
+    const uint32_t x = c / plane;
+    return ((uint64_t)x * kQ + i) * plane + (c - x * plane);
+  }
   int lx, ly, lz;
   uint32_t plane;        // ly*lz
   uint32_t ncell;
@@ -341,38 +347,7 @@ struct Tiling {
   int xcd_remap;
 };
 
-// Pull the 19 post-stream populations of one fluid cell (stream + boundary,
-// iteration_kernel.cuh:51-118).  `src(i)` is the upstream cell index for
-// direction i, `self` the cell's own index.
-template <class Mo, class SrcFn>
-__device__ __forceinline__ void pull(const Lattice<Mo>& L, const Phys<typename Mo::C>& ph, uint32_t lk,
-                                     uint32_t self, SrcFn src, typename Mo::C* f) {
-  f[0] = decode<Mo>(L.fin[self], 0, ph);
-#pragma unroll
-  for (int i = 1; i < kQ; ++i) {
-    const bool bb = (lk >> i) & 1u;
-    const typename Mo::S* p = bb ? (L.fin + (uint64_t)opp(i) * L.pstride + self)
-                                 : (L.fin + (uint64_t)i * L.pstride + src(i));
-    // a bounce-back direction reads population opp(i) of this cell but
-    // decodes with the weight of opp(i), which equals the weight of i
-    f[i] = decode<Mo>(*p, i, ph);
-  }
-}
-
-// Face rule applied after the pull on a face cell (rare, divergent path).
-template <class Mo>
-__device__ __noinline__ void apply_faces(const Lattice<Mo>& L, const Phys<typename Mo::C>& ph, uint32_t self,
-                                         int x, int y, int z, typename Mo::C* f) {
-  using C = typename Mo::C;
-  C a[kQ];
-#pragma unroll
-  for (int k = 0; k < kQ; ++k) a[k] = decode<Mo>(L.fin[(uint64_t)k * L.pstride + self], k, ph);
-#pragma unroll
-  for (int i = 1; i < kQ; ++i) {
-    const int fc = entering_face(ph, i, x, y, z, L.lx, L.ly, L.lz);
-    if (fc >= 0) f[i] = abb_value<Mo>(a, i, ph.face_rho[fc]);
-  }
-}
+__device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
 template <class C>
 __device__ __forceinline__ bool on_face(const Phys<C>& ph, int x, int y, int z, int lx, int ly, int lz) {
@@ -380,14 +355,74 @@ __device__ __forceinline__ bool on_face(const Phys<C>& ph, int x, int y, int z, 
          (ph.face[3] && y == ly - 1) || (ph.face[4] && z == 0) || (ph.face[5] && z == lz - 1);
 }
 
-__device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+// Global loads/stores at a 32-bit byte offset from a wave-uniform base: the
+// form that maps to `global_load/store ... v_off, s[base:base+1]`.
+template <class S>
+__device__ __forceinline__ S ld(const S* base, uint32_t off_bytes) {
+  return *reinterpret_cast<const S*>(reinterpret_cast<const char*>(base) + off_bytes);
+}
+template <class S>
+__device__ __forceinline__ void st(S* base, uint32_t off_bytes, S v) {
+  *reinterpret_cast<S*>(reinterpret_cast<char*>(base) + off_bytes) = v;
+}
+
+// Face rule after the pull on a face cell: each direction entering through a
+// PRESSURE face takes the anti-bounce-back value built from the cell's own
+// populations of the previous step.
+template <class Mo>
+__device__ __forceinline__ void apply_faces(const Lattice<Mo>& L, const Phys<typename Mo::C>& ph, uint32_t self,
+                                            int x, int y, int z, typename Mo::C* f) {
+  using C = typename Mo::C;
+  C a[kQ];
+#pragma unroll
+  for (int k = 0; k < kQ; ++k) a[k] = decode<Mo>(L.fin[L.pix(k, self)], k, ph);
+#pragma unroll
+  for (int i = 1; i < kQ; ++i) {
+    const int fc = entering_face(ph, i, x, y, z, L.lx, L.ly, L.lz);
+    if (fc >= 0) f[i] = abb_value<Mo>(a, i, ph.face_rho[fc]);
+  }
+}
+
+// Halfway bounce-back is resolved on the write side.  After a fluid cell x
+// collides it stores f*_i(x) in its own slots and, for every direction i
+// whose upstream node x - e_i is solid, also stores f*_opp(i)(x) in slot i
+// of that solid node.  The next step's pull of direction i at x reads slot i
+// of x - e_i unconditionally and so receives exactly the bounce-back value
+// B_i(x) = A_opp(i)(x) of iteration_kernel.cuh:112-115.  A solid slot (i, s)
+// is written only by the unique fluid node s + e_i, and solid nodes are never
+// updated otherwise, so the extra stores do not race.
+template <class P, class Q>
+struct PtrOff {
+  P first;
+  Q second;
+};
+
+template <class Mo, class OwnFn, class UpFn>
+__device__ __forceinline__ void store_cell(uint32_t lk, OwnFn own, UpFn up, const typename Mo::C* f,
+                                           const Phys<typename Mo::C>& ph) {
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) {
+    const auto o = own(i);
+    st(o.first, o.second, encode<Mo>(f[i], i, ph));
+  }
+  if (lk & ~1u) {
+#pragma unroll
+    for (int i = 1; i < kQ; ++i)
+      if ((lk >> i) & 1u) {
+        const auto o = up(i);
+        st(o.first, o.second, encode<Mo>(f[opp(i)], i, ph));
+      }
+  }
+}
+
 
 // ----------------------------------------------------------------------------
 // fused marching stream-collide kernel
 
-template <class Mo>
-__global__ __launch_bounds__(1024) void k_fused(Lattice<Mo> L, Phys<typename Mo::C> ph, Tiling T) {
+template <class Mo, bool kFaces, int kBlock>
+__global__ __launch_bounds__(kBlock) void k_fused(Lattice<Mo> L, Phys<typename Mo::C> ph, Tiling T) {
   using C = typename Mo::C;
+  using S = typename Mo::S;
   extern __shared__ unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
 
@@ -439,14 +474,15 @@ __global__ __launch_bounds__(1024) void k_fused(Lattice<Mo> L, Phys<typename Mo:
   }
   const bool interior = iny && inz;
 
-  // (y,z) offsets of the 9 in-plane neighbours, index [dy+1][dz+1]
-  uint32_t yzo[3][3];
+  // byte offsets within an x-plane of the 9 in-plane neighbours, [dy+1][dz+1]
+  uint32_t yzb[3][3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const uint32_t row = (uint32_t)wrapi(gy + a - 1, L.ly) * (uint32_t)L.lz;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) yzo[a][c] = row + (uint32_t)wrapi(gz + c - 1, L.lz);
+    for (int c = 0; c < 3; ++c) yzb[a][c] = (row + (uint32_t)wrapi(gz + c - 1, L.lz)) * (uint32_t)sizeof(S);
   }
+  const uint32_t self_yz = yzb[1][1] / (uint32_t)sizeof(S);
   const int lrow[3] = {rowm, ty, rowp};
   const int lcol[3] = {colm, tz, colp};
 
@@ -458,18 +494,19 @@ __global__ __launch_bounds__(1024) void k_fused(Lattice<Mo> L, Phys<typename Mo:
   auto load_plane = [&](int xp, C* f, uint32_t& lk) -> C {
     const int xm = xp == 0 ? L.lx - 1 : xp - 1;
     const int xq = xp + 1 == L.lx ? 0 : xp + 1;
-    const uint32_t p0 = (uint32_t)xp * L.plane, pm = (uint32_t)xm * L.plane, pq = (uint32_t)xq * L.plane;
-    const uint32_t self = p0 + yzo[1][1];
-    lk = thr ? L.link[self] : 1u;
+    const uint64_t p0 = (uint64_t)xp * L.plane;
+    lk = thr ? L.link[p0 + self_yz] : 1u;
     C r = (C)0;
     if (!(lk & 1u)) {
-      pull<Mo>(L, ph, lk, self,
-               [&](int i) -> uint32_t {
-                 const uint32_t pl = ex(i) == -1 ? pq : (ex(i) == 1 ? pm : p0);
-                 return pl + yzo[1 - ey(i)][1 - ez(i)];
-               },
-               f);
-      if (ph.any_face && on_face(ph, xp, gy, gz, L.lx, L.ly, L.lz)) apply_faces<Mo>(L, ph, self, xp, gy, gz, f);
+#pragma unroll
+      for (int i = 0; i < kQ; ++i) {
+        const uint64_t xs = ex(i) == -1 ? xq : (ex(i) == 1 ? xm : xp);
+        const S* base = L.fin + (xs * kQ + i) * L.plane;
+        f[i] = decode<Mo>(ld(base, yzb[1 - ey(i)][1 - ez(i)]), i, ph);
+      }
+      if constexpr (kFaces) {
+        if (on_face(ph, xp, gy, gz, L.lx, L.ly, L.lz)) apply_faces<Mo>(L, ph, (uint32_t)p0 + self_yz, xp, gy, gz, f);
+      }
       r = density(f, gy, ph);
     }
     return r;
@@ -496,9 +533,12 @@ __global__ __launch_bounds__(1024) void k_fused(Lattice<Mo> L, Phys<typename Mo:
     __syncthreads();
 
     if (interior && !(lkP & 1u)) {
-      const uint32_t self = (uint32_t)x * L.plane + yzo[1][1];
+      const uint64_t p0 = (uint64_t)x * L.plane;
+      const uint64_t xm = (uint64_t)(x == 0 ? L.lx - 1 : x - 1);
       C F[3];
-      if (ph.any_face && force_skipped(ph, x, gy, gz, L.lx, L.ly, L.lz)) {
+      bool skip = false;
+      if constexpr (kFaces) skip = force_skipped(ph, x, gy, gz, L.lx, L.ly, L.lz);
+      if (skip) {
         F[0] = F[1] = F[2] = (C)0;
       } else {
         C pn[kQ];
@@ -512,9 +552,17 @@ __global__ __launch_bounds__(1024) void k_fused(Lattice<Mo> L, Phys<typename Mo:
       }
       C u[3];
       collide<Mo>(P, rhoP, F, ph, u);
-#pragma unroll
-      for (int i = 0; i < kQ; ++i) L.fout[(uint64_t)i * L.pstride + self] = encode<Mo>(P[i], i, ph);
+      store_cell<Mo>(lkP,
+                     [&](int i) {
+                       return PtrOff<S*, uint32_t>{L.fout + ((uint64_t)x * kQ + i) * L.plane, yzb[1][1]};
+                     },
+                     [&](int i) {
+                       const uint64_t xs = ex(i) == -1 ? (uint64_t)xn : (ex(i) == 1 ? xm : (uint64_t)x);
+                       return PtrOff<S*, uint32_t>{L.fout + (xs * kQ + i) * L.plane, yzb[1 - ey(i)][1 - ez(i)]};
+                     },
+                     P, ph);
       if (ph.write_macro) {
+        const uint64_t self = p0 + self_yz;
         L.rho[self] = (typename Mo::M)rhoP;
         L.pres[self] = (typename Mo::M)pressure_of<Mo>(ph, rhoP);
         L.u[self] = (typename Mo::M)u[0];
@@ -548,9 +596,10 @@ __device__ __forceinline__ uint32_t cell_at(const Lattice<Mo>& L, int x, int y, 
 
 template <class Mo>
 __device__ __forceinline__ typename Mo::C pull_cell(const Lattice<Mo>& L, const Phys<typename Mo::C>& ph,
-                                                   uint32_t c, uint32_t lk, int x, int y, int z,
-                                                   typename Mo::C* f) {
-  pull<Mo>(L, ph, lk, c, [&](int i) -> uint32_t { return cell_at(L, x - ex(i), y - ey(i), z - ez(i)); }, f);
+                                                   uint32_t c, int x, int y, int z, typename Mo::C* f) {
+#pragma unroll
+  for (int i = 0; i < kQ; ++i)
+    f[i] = decode<Mo>(L.fin[L.pix(i, cell_at(L, x - ex(i), y - ey(i), z - ez(i)))], i, ph);
   if (ph.any_face && on_face(ph, x, y, z, L.lx, L.ly, L.lz)) apply_faces<Mo>(L, ph, c, x, y, z, f);
   return density(f, y, ph);
 }
@@ -565,7 +614,7 @@ __global__ __launch_bounds__(256) void k_density(Lattice<Mo> L, Phys<typename Mo
   int x, y, z;
   cell_coords(L, c, x, y, z);
   C f[kQ];
-  const C rho = pull_cell(L, ph, c, lk, x, y, z, f);
+  const C rho = pull_cell(L, ph, c, x, y, z, f);
   L.psi[c] = psi_of<Mo>(ph, rho);
   if (ph.write_macro) {
     L.rho[c] = (typename Mo::M)rho;
@@ -576,6 +625,7 @@ __global__ __launch_bounds__(256) void k_density(Lattice<Mo> L, Phys<typename Mo
 template <class Mo>
 __global__ __launch_bounds__(256) void k_collide(Lattice<Mo> L, Phys<typename Mo::C> ph) {
   using C = typename Mo::C;
+  using S = typename Mo::S;
   const uint32_t c = blockIdx.x * 256u + threadIdx.x;
   if (c >= L.ncell) return;
   const uint32_t lk = L.link[c];
@@ -583,7 +633,7 @@ __global__ __launch_bounds__(256) void k_collide(Lattice<Mo> L, Phys<typename Mo
   int x, y, z;
   cell_coords(L, c, x, y, z);
   C f[kQ];
-  const C rho = pull_cell(L, ph, c, lk, x, y, z, f);
+  const C rho = pull_cell(L, ph, c, x, y, z, f);
   C F[3];
   if (ph.any_face && force_skipped(ph, x, y, z, L.lx, L.ly, L.lz)) {
     F[0] = F[1] = F[2] = (C)0;
@@ -596,13 +646,35 @@ __global__ __launch_bounds__(256) void k_collide(Lattice<Mo> L, Phys<typename Mo
   }
   C u[3];
   collide<Mo>(f, rho, F, ph, u);
-#pragma unroll
-  for (int i = 0; i < kQ; ++i) L.fout[(uint64_t)i * L.pstride + c] = encode<Mo>(f[i], i, ph);
+  store_cell<Mo>(lk, [&](int i) { return PtrOff<S*, uint32_t>{L.fout + L.pix(i, c), 0u}; },
+                 [&](int i) {
+                   return PtrOff<S*, uint32_t>{L.fout + L.pix(i, cell_at(L, x - ex(i), y - ey(i), z - ez(i))), 0u};
+                 },
+                 f, ph);
   if (ph.write_macro) {
     L.u[c] = (typename Mo::M)u[0];
     L.u[(uint64_t)L.ncell + c] = (typename Mo::M)u[1];
     L.u[2ull * L.ncell + c] = (typename Mo::M)u[2];
   }
+}
+
+// Fill the bounce-back slots of solid nodes from the fluid populations of a
+// lattice (after initialisation and after a population write), as the
+// collision kernels do after every step.
+template <class Mo>
+__global__ void k_ghost(Lattice<Mo> L) {
+  using S = typename Mo::S;
+  const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+  if (c >= L.ncell) return;
+  const uint32_t lk = L.link[c];
+  if ((lk & 1u) || !(lk & ~1u)) return;
+  int x, y, z;
+  cell_coords(L, c, x, y, z);
+  S* lat = L.fout;
+#pragma unroll
+  for (int i = 1; i < kQ; ++i)
+    if ((lk >> i) & 1u)
+      lat[L.pix(i, cell_at(L, x - ex(i), y - ey(i), z - ez(i)))] = lat[L.pix(opp(i), c)];
 }
 
 // ----------------------------------------------------------------------------
@@ -639,8 +711,8 @@ __global__ void k_init(Lattice<Mo> L, typename Mo::S* fother, Phys<typename Mo::
 #pragma unroll
   for (int i = 0; i < kQ; ++i) {
     const typename Mo::S v = encode<Mo>(weight<C>(i) * rho, i, ph);
-    L.fout[(uint64_t)i * L.pstride + c] = v;
-    fother[(uint64_t)i * L.pstride + c] = v;
+    L.fout[L.pix(i, c)] = v;
+    fother[L.pix(i, c)] = v;
   }
   if (fluid) {
     L.rho[c] = (typename Mo::M)rho;
@@ -662,7 +734,7 @@ __global__ void k_read_pops(Lattice<Mo> L, Phys<typename Mo::C> ph, double* out)
   const uint32_t c = blockIdx.x * 256u + threadIdx.x;
   if (c >= L.ncell) return;
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) out[(uint64_t)c * kQ + i] = (double)decode<Mo>(L.fin[(uint64_t)i * L.pstride + c], i, ph);
+  for (int i = 0; i < kQ; ++i) out[(uint64_t)c * kQ + i] = (double)decode<Mo>(L.fin[L.pix(i, c)], i, ph);
 }
 
 template <class Mo>
@@ -671,7 +743,7 @@ __global__ void k_write_pops(Lattice<Mo> L, Phys<typename Mo::C> ph, const doubl
   const uint32_t c = blockIdx.x * 256u + threadIdx.x;
   if (c >= L.ncell) return;
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) L.fout[(uint64_t)i * L.pstride + c] = encode<Mo>((C)in[(uint64_t)c * kQ + i], i, ph);
+  for (int i = 0; i < kQ; ++i) L.fout[L.pix(i, c)] = encode<Mo>((C)in[(uint64_t)c * kQ + i], i, ph);
 }
 
 // ----------------------------------------------------------------------------
@@ -693,7 +765,7 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(Lattice<Mo> L, Phys<type
     double s = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
 #pragma unroll
     for (int i = 0; i < kQ; ++i) {
-      const double f = (double)decode<Mo>(L.fin[(uint64_t)i * L.pstride + c], i, ph);
+      const double f = (double)decode<Mo>(L.fin[L.pix(i, c)], i, ph);
       if (!(f - f == 0.0)) acc[15] += 1.0;
       s += f;
       mx += f * ex(i);
@@ -745,7 +817,7 @@ struct mplbm_ctx {
   mplbm_config cfg{};
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  uint64_t ncell = 0, pstride = 0;
+  uint64_t ncell = 0;
   int bytes_pop = 4;
   void* lat[2] = {nullptr, nullptr};  // population lattices
   int cur = 0;                        // lattice holding the newest post-collision state
@@ -840,7 +912,6 @@ static Lattice<Mo> make_lattice(const mplbm_ctx* ctx, int in, int out) {
   L.pres = static_cast<typename Mo::M*>(ctx->pres);
   L.u = static_cast<typename Mo::M*>(ctx->u);
   L.psi = static_cast<typename Mo::C*>(ctx->psi);
-  L.pstride = ctx->pstride;
   L.lx = (int)ctx->cfg.lx;
   L.ly = (int)ctx->cfg.ly;
   L.lz = (int)ctx->cfg.lz;
@@ -858,7 +929,7 @@ static mplbm_status choose_tiling(mplbm_ctx* ctx) {
   if (tz <= 0) tz = c.lz <= 64 ? (int)c.lz : 34;
   if (tz >= c.lz) { T.wrap_z = 1; tz = (int)c.lz; }
   if (ty <= 0) {
-    ty = std::max(1, 1024 / tz);
+    ty = std::max(1, 384 / tz);
     if (ty > 32) ty = 32;
   }
   if (ty >= c.ly && (int64_t)tz * c.ly <= 1024) { T.wrap_y = 1; ty = (int)c.ly; }
@@ -870,7 +941,12 @@ static mplbm_status choose_tiling(mplbm_ctx* ctx) {
   T.TZ = tz;
   T.tiles_y = T.wrap_y ? 1 : (int)((c.ly + (ty - 2) - 1) / (ty - 2));
   T.tiles_z = T.wrap_z ? 1 : (int)((c.lz + (tz - 2) - 1) / (tz - 2));
-  if (seg <= 0) seg = 16;
+  if (seg <= 0) {
+    // long segments amortise the two warm-up planes; keep >= ~2 blocks per CU
+    const int64_t tiles = (int64_t)T.tiles_y * T.tiles_z;
+    seg = 64;
+    while (seg > 4 && tiles * ((c.lx + seg - 1) / seg) < 1024) seg /= 2;
+  }
   if (seg > c.lx) seg = (int)c.lx;
   T.seg = seg;
   T.nseg = (int)((c.lx + seg - 1) / seg);
@@ -894,7 +970,24 @@ static mplbm_status launch_step(mplbm_ctx* ctx, int64_t step, int32_t reverse, u
     const int threads = fused_threads(T);
     const int grid = T.xcd_remap ? ((T.nblocks + 7) / 8) * 8 : T.nblocks;
     const size_t shm = 4 * (size_t)T.TY * T.TZ * sizeof(C);
-    hipLaunchKernelGGL(k_fused<Mo>, dim3(grid), dim3(threads), shm, ctx->stream, L, ph, T);
+    // the launch bound sets the register budget: 1024-thread tiles cap a
+    // thread at 128 VGPRs, 512 and 256 let the compiler keep every value in
+    // registers at 3 waves per SIMD
+    if (ph.any_face) {
+      if (threads > 512)
+        hipLaunchKernelGGL((k_fused<Mo, true, 1024>), dim3(grid), dim3(threads), shm, ctx->stream, L, ph, T);
+      else if (threads > 256)
+        hipLaunchKernelGGL((k_fused<Mo, true, 512>), dim3(grid), dim3(threads), shm, ctx->stream, L, ph, T);
+      else
+        hipLaunchKernelGGL((k_fused<Mo, true, 256>), dim3(grid), dim3(threads), shm, ctx->stream, L, ph, T);
+    } else {
+      if (threads > 512)
+        hipLaunchKernelGGL((k_fused<Mo, false, 1024>), dim3(grid), dim3(threads), shm, ctx->stream, L, ph, T);
+      else if (threads > 256)
+        hipLaunchKernelGGL((k_fused<Mo, false, 512>), dim3(grid), dim3(threads), shm, ctx->stream, L, ph, T);
+      else
+        hipLaunchKernelGGL((k_fused<Mo, false, 256>), dim3(grid), dim3(threads), shm, ctx->stream, L, ph, T);
+    }
   } else {
     const int grid = (int)((ctx->ncell + 255) / 256);
     hipLaunchKernelGGL(k_density<Mo>, dim3(grid), dim3(256), 0, ctx->stream, L, ph);
@@ -958,9 +1051,6 @@ mplbm_status mplbm_create(const mplbm_config* cfg, mplbm_ctx** out) {
   ctx->cfg = *cfg;
   ctx->ncell = n;
   ctx->bytes_pop = (int)storage_bytes(cfg->storage);
-  // population arrays start on 256-byte boundaries
-  const uint64_t align_elems = 256 / ctx->bytes_pop;
-  ctx->pstride = (n + align_elems - 1) / align_elems * align_elems;
   ctx->kernel = cfg->kernel == MPLBM_KERNEL_TWOPASS ? MPLBM_KERNEL_TWOPASS : MPLBM_KERNEL_FUSED;
   mplbm_status st = choose_tiling(ctx);
   if (st != MPLBM_OK) {
@@ -984,7 +1074,7 @@ mplbm_status mplbm_create(const mplbm_config* cfg, mplbm_ctx** out) {
   CREATE_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   CREATE_TRY(hipEventCreate(&ctx->ev0));
   CREATE_TRY(hipEventCreate(&ctx->ev1));
-  const size_t lat_bytes = (size_t)ctx->pstride * kQ * ctx->bytes_pop;
+  const size_t lat_bytes = (size_t)n * kQ * ctx->bytes_pop;
   const size_t mb = cfg->storage == MPLBM_FP64 ? 8 : 4;  // macro / psi element size
   CREATE_TRY(hipMalloc(&ctx->lat[0], lat_bytes));
   CREATE_TRY(hipMalloc(&ctx->lat[1], lat_bytes));
@@ -1080,6 +1170,7 @@ mplbm_status mplbm_init_fields_f64(mplbm_ctx* ctx, const double* rho) {
       auto L = make_lattice<Mo>(ctx, 0, 0);
       hipLaunchKernelGGL(k_init<Mo>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, L,
                          static_cast<typename Mo::S*>(ctx->lat[1]), ph, d_rho);
+      hipLaunchKernelGGL(k_ghost<Mo>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, L);
       return MPLBM_OK;
     });
     (void)st;
@@ -1220,6 +1311,7 @@ mplbm_status mplbm_write_populations_f64(mplbm_ctx* ctx, const double* f) {
       const auto ph = make_phys<typename Mo::C>(ctx, 0, 1, 0, false);
       auto L = make_lattice<Mo>(ctx, ctx->cur, ctx->cur);
       hipLaunchKernelGGL(k_write_pops<Mo>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, L, ph, d);
+      hipLaunchKernelGGL(k_ghost<Mo>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, L);
       return MPLBM_OK;
     });
     e = hipGetLastError();
@@ -1266,6 +1358,21 @@ mplbm_status mplbm_reduce(mplbm_ctx* ctx, mplbm_stats64* out) {
   out->p_max = h[14];
   out->nonfinite = h[15];
   return MPLBM_OK;
+}
+
+mplbm_status mplbm_set_tiling(mplbm_ctx* ctx, int32_t tile_y, int32_t tile_z, int32_t seg_x) {
+  if (!ctx) return fail(ctx, MPLBM_ERR_ARG, "null context");
+  const mplbm_config saved = ctx->cfg;
+  const Tiling saved_t = ctx->tiling;
+  ctx->cfg.tile_y = tile_y;
+  ctx->cfg.tile_z = tile_z;
+  ctx->cfg.seg_x = seg_x;
+  mplbm_status st = choose_tiling(ctx);
+  if (st != MPLBM_OK) {
+    ctx->cfg = saved;
+    ctx->tiling = saved_t;
+  }
+  return st;
 }
 
 mplbm_status mplbm_info_get(const mplbm_ctx* ctx, mplbm_info* out) {
