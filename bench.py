@@ -41,13 +41,17 @@ def parse():
     ap.add_argument("--rmin", type=float, default=6.0)
     ap.add_argument("--rmax", type=float, default=18.0)
     ap.add_argument("--seed", type=int, default=3)
-    ap.add_argument("--storage", default="fp32,fp16s,bf16s")
+    ap.add_argument("--storage", default="fp32,fp32fast,fp16s,bf16s")
     ap.add_argument("--kernel", default="fused")
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--tile-z", type=int, default=0)
     ap.add_argument("--seg-x", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline (0: skip)")
     ap.add_argument("--profile-only", action="store_true", help="run only the FP32 timed loop (for rocprofv3)")
+    ap.add_argument("--precision", action="store_true", help="error study of FP32/FP16S/BF16S against FP64")
+    ap.add_argument("--precision-steps", type=int, default=1000)
+    ap.add_argument("--slab-steps", type=int, default=3000)
+    ap.add_argument("--perm-steps", type=int, default=3000)
     ap.add_argument("--sweep", action="store_true", help="time every autotuner tile candidate (FP32) and exit")
     ap.add_argument("--autotune", action="store_true", help="pick the tile with the autotuner before timing")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -94,7 +98,7 @@ def run_gpu(a, shape, fluid, rho, storage):
     nf = info["fluid_nodes"]
     launches = a.steps * (1 if a.kernel == "fused" else 2)
     per_launch_s = ev_ms / 1e3 / launches
-    bpop = mp.STORAGE_BYTES[mp.STORAGE_NAMES[storage]]
+    bpop = mp.STORAGE_BYTES[mp.STORAGE_NAMES[storage.replace("fast", "")]]
     alg_bytes = 2 * Q * bpop * nf  # per step
     return {
         "mlups": nf * a.steps / wall / 1e6,
@@ -139,6 +143,94 @@ def traffic_from_profiles(storage):
         return None
 
 
+def precision_study(a, shape, fluid, rho):
+    """Error of the 16-bit storage forms (and FP32) against FP64 on the same
+    packing.  The FP64 GPU path evaluates the oracle's FP64 operation
+    sequence (bit-identical in tests/test_gpu_parity.py), so its fields stand
+    for the FP64 oracle at sizes the CPU cannot run.  Prints one JSON line."""
+    import multiphase_lbm_amd as mp
+    out = {"workload": f"{shape[0]}^3 packing, porosity {fluid.mean():.3f}"}
+    fl = fluid.astype(bool)
+
+    def run(storage, params, steps, rho0, **kw):
+        with mp.Context(*shape, storage=storage, kernel="fused", **params, **kw) as ctx:
+            ctx.set_geometry(fluid)
+            ctx.init_fields(rho0)
+            t = time.perf_counter()
+            ctx.step(0, steps, write_macro=True)
+            r, p, u = ctx.read_fields()
+            dt = time.perf_counter() - t
+            st = ctx.reduce()
+            info = ctx.info()
+        return r, u, st, dt, info
+
+    # (1) two-phase Shan-Chen run, G = -5
+    steps = a.precision_steps
+    ref_r, _, ref_st, _, _ = run("fp64", PHYS, steps, rho.astype(np.float64))
+    sat_thr = 0.5 * (1.932442 + 0.156413)
+    two = {"steps": steps, "fp64": {"saturation": float((ref_r[fl] > sat_thr).mean()),
+                                    "mass_rel_drift": ref_st["mass"] / float(rho.astype(np.float64)[fl].sum()) - 1}}
+    variants = [("fp32", {}), ("fp16s", {}), ("fp16s", {"rho_ref": 1.04}), ("bf16s", {}), ("bf16s", {"rho_ref": 1.04})]
+    for storage, kw in variants:
+        r, _, st, dt, info = run(storage, PHYS, steps, rho, **kw)
+        key = storage + ("" if not kw else f"_rhoref{kw['rho_ref']}")
+        two[key] = {
+            "density_rel_l2": float(np.linalg.norm(r[fl] - ref_r[fl]) / np.linalg.norm(ref_r[fl])),
+            "density_max_abs": float(np.abs(r[fl] - ref_r[fl]).max()),
+            "saturation": float((r[fl] > sat_thr).mean()),
+            "mass_rel_drift_fp64_sum": st["mass"] / float(rho.astype(np.float64)[fl].sum()) - 1,
+            "rho_ref": info["rho_ref"], "scale": info["dev_scale"], "nonfinite": st["nonfinite"]}
+        print(f"# two-phase {key}: {two[key]}", file=sys.stderr, flush=True)
+    out["two_phase_G-5"] = two
+
+    # (2) flat interface: liquid slab in vapour, 128x16x16 periodic
+    sshape = (128, 16, 16)
+    xs = np.arange(128)
+    slab = np.where((xs >= 32) & (xs < 96), 1.9, 0.16)[:, None, None] * np.ones(sshape)
+    sfl = np.ones(sshape, np.uint8)
+
+    def run_slab(storage, **kw):
+        with mp.Context(*sshape, storage=storage, kernel="fused", tau=1.0, G=-5.0, rhos=0.7, **kw) as ctx:
+            ctx.set_geometry(sfl)
+            ctx.init_fields(slab.astype(np.float64 if storage == "fp64" else np.float32))
+            ctx.step(0, a.slab_steps, write_macro=True)
+            r, _, _ = ctx.read_fields()
+        prof = r.mean(axis=(1, 2))
+        mid = 0.5 * (prof.max() + prof.min())
+        # first up-crossing of the midpoint density, linear interpolation
+        k = int(np.argmax((prof[:-1] < mid) & (prof[1:] >= mid)))
+        pos = k + (mid - prof[k]) / (prof[k + 1] - prof[k])
+        return {"rho_l": float(prof[64]), "rho_v": float(prof[0]), "interface_x": float(pos)}
+
+    sl = {"steps": a.slab_steps, "fp64": run_slab("fp64")}
+    for storage, kw in variants:
+        key = storage + ("" if not kw else f"_rhoref{kw['rho_ref']}")
+        v = run_slab(storage, **kw)
+        v["rho_l_rel_err"] = abs(v["rho_l"] - sl["fp64"]["rho_l"]) / sl["fp64"]["rho_l"]
+        v["rho_v_rel_err"] = abs(v["rho_v"] - sl["fp64"]["rho_v"]) / sl["fp64"]["rho_v"]
+        v["interface_shift"] = v["interface_x"] - sl["fp64"]["interface_x"]
+        sl[key] = v
+    out["flat_interface"] = sl
+
+    # (3) Darcy permeability: single phase G = 0, tau = 0.8, g = 1e-6 along x
+    tau, g = 0.8, 1e-6
+    nu = (tau - 0.5) / 3
+    perm = {"steps": a.perm_steps, "tau": tau, "g": g}
+    ones = np.ones(shape)
+    for storage in ("fp64", "fp32", "fp16s", "bf16s"):
+        r, u, st, dt, info = run(storage, dict(tau=tau, G=0.0, rhos=1.0, body_force=(g, 0.0, 0.0)), a.perm_steps,
+                                 ones.astype(np.float64 if storage == "fp64" else np.float32))
+        # physical velocity (m + F/2)/rho = shifted u - (tau - 1/2) g; superficial average over all cells
+        ux = np.where(fl, u[..., 0] - (tau - 0.5) * g, 0.0)
+        k = nu * float(ux.mean()) / g
+        perm[storage] = {"k_lattice_units": k, "seconds": round(dt, 2)}
+        if storage != "fp64":
+            perm[storage]["rel_err"] = abs(k - perm["fp64"]["k_lattice_units"]) / perm["fp64"]["k_lattice_units"]
+        print(f"# permeability {storage}: {perm[storage]}", file=sys.stderr, flush=True)
+    out["permeability"] = perm
+    print(json.dumps(out), flush=True)
+
+
 def copy_calibration():
     """A plain copy kernel of known size: FETCH_SIZE/WRITE_SIZE of this run
     against the 2 GiB read + 2 GiB written give the counter scale."""
@@ -162,6 +254,9 @@ def main():
               "a design note, not built", file=sys.stderr)
         sys.exit(2)
     shape, fluid, rho, nsph = workload(a)
+    if a.precision:
+        precision_study(a, shape, fluid, rho)
+        return
     if a.sweep:
         import multiphase_lbm_amd as mp
         from multiphase_lbm_amd.autotune import autotune
@@ -175,7 +270,7 @@ def main():
                     print(json.dumps({"storage": storage, "tile_y": ty, "tile_z": tz, "seg_x": seg,
                                       "ms": round(ms, 4), "mlups": round(nf / ms / 1e3, 1)}), flush=True)
         return
-    storages = ["fp32"] if a.profile_only else a.storage.split(",")
+    storages = a.storage.split(",")[:1] if a.profile_only else a.storage.split(",")
     res = {}
     for s in storages:
         res[s] = run_gpu(a, shape, fluid, rho, s)

@@ -61,6 +61,12 @@ typedef enum mplbm_kernel {
   MPLBM_KERNEL_TWOPASS = 2
 } mplbm_kernel;
 
+/* Arithmetic of the FP32 form.  REFERENCE evaluates the reference's
+ * operation sequence, including the double-promoted sub-expressions C++ gives
+ * it (iteration_device.cuh:7-206); FAST evaluates the same model in FP32 with
+ * fused multiply-adds and hardware reciprocal/exp. */
+typedef enum mplbm_math { MPLBM_MATH_REFERENCE = 0, MPLBM_MATH_FAST = 1 } mplbm_math;
+
 /* Outer-face types (index order xmin, xmax, ymin, ymax, zmin, zmax).  The
  * reference's face flags (include/global_var.cuh:39-63) are fixed to
  * periodic; PRESSURE prescribes face_rho[f] through the anti-bounce-back
@@ -77,7 +83,7 @@ typedef struct mplbm_config {
   double tau;                  /* BGK relaxation time */
   double G;                    /* Shan-Chen interaction strength */
   int32_t carn_star;           /* 1: Carnahan-Starling EOS, 0: Shan-Chen */
-  int32_t reserved0;
+  int32_t math;                /* mplbm_math (FP32 storage only; 16-bit forms are FAST) */
   double T_Tc;                 /* CS EOS T/Tc */
   double rhol_spinodal;        /* liquid threshold (injection types 1/2, stats) */
   double rhog_spinodal;        /* vapour threshold (stats) */

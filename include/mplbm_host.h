@@ -45,6 +45,7 @@ typedef struct mplbm_input {
   /* build extensions */
   int32_t storage;          /* mplbm_storage */
   int32_t kernel;           /* mplbm_kernel */
+  int32_t math;             /* mplbm_math */
   int32_t face_type[6];
   double face_rho[6];
   double body_force[3];

@@ -43,12 +43,15 @@ class Context:
     Keyword parameters are fields of mplbm_config (tau, G, carn_star, T_Tc,
     rhol_spinodal, rhog_spinodal, rho_inject_period, rho_increment, rhos,
     inject_position, face_type, face_rho, body_force, rho_ref, dev_scale,
-    tile_y, tile_z, seg_x)."""
+    tile_y, tile_z, seg_x, math).  storage "fp32fast" is FP32 storage with
+    math="fast"."""
 
     def __init__(self, lx, ly, lz, storage="fp32", kernel="auto", **params):
         self.lib = _abi.load()
         cfg = default_config()
         cfg.lx, cfg.ly, cfg.lz = int(lx), int(ly), int(lz)
+        if storage == "fp32fast":
+            storage, params = "fp32", dict(params, math="fast")
         cfg.storage = STORAGE_NAMES[storage] if isinstance(storage, str) else int(storage)
         cfg.kernel = KERNEL_NAMES[kernel] if isinstance(kernel, str) else int(kernel)
         for k, v in params.items():
@@ -58,6 +61,8 @@ class Context:
                     arr[i] = x
             elif k == "carn_star":
                 cfg.carn_star = int(bool(v))
+            elif k == "math":
+                cfg.math = {"reference": 0, "fast": 1}[v] if isinstance(v, str) else int(v)
             else:
                 if not hasattr(cfg, k):
                     raise TypeError(f"unknown parameter {k}")

@@ -15,6 +15,7 @@ OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_NOMEM, ERR_UNSUPPORTED = range(6)
 FP32, FP64, FP16S, BF16S = range(4)
 KERNEL_AUTO, KERNEL_FUSED, KERNEL_TWOPASS = range(3)
 FACE_PERIODIC, FACE_PRESSURE = 0, 1
+MATH_REFERENCE, MATH_FAST = 0, 1
 STEP_WRITE_MACRO = 1
 
 STORAGE_NAMES = {"fp32": FP32, "fp64": FP64, "fp16s": FP16S, "bf16s": BF16S}
@@ -26,7 +27,7 @@ class Config(ctypes.Structure):
         ("lx", ctypes.c_int64), ("ly", ctypes.c_int64), ("lz", ctypes.c_int64),
         ("storage", ctypes.c_int32), ("kernel", ctypes.c_int32),
         ("tau", ctypes.c_double), ("G", ctypes.c_double),
-        ("carn_star", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+        ("carn_star", ctypes.c_int32), ("math", ctypes.c_int32),
         ("T_Tc", ctypes.c_double), ("rhol_spinodal", ctypes.c_double), ("rhog_spinodal", ctypes.c_double),
         ("rho_inject_period", ctypes.c_int64), ("rho_increment", ctypes.c_double),
         ("rhos", ctypes.c_double), ("inject_position", ctypes.c_double),

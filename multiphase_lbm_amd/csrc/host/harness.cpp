@@ -545,6 +545,7 @@ struct MultiphaseLBM {
     cfg.lx = d.lx; cfg.ly = d.ly; cfg.lz = d.lz;
     cfg.storage = in.storage;
     cfg.kernel = in.kernel;
+    cfg.math = in.math;
     cfg.tau = (float)in.tau;
     cfg.G = (float)in.G;
     cfg.carn_star = in.carn_star;
@@ -675,6 +676,9 @@ bool MultiphaseLBM::mplbm_parse_scalar_token(std::istream& file, const std::stri
   else if (token == "storage") {
     file >> s;
     in.storage = s == "fp64" ? MPLBM_FP64 : s == "fp16s" ? MPLBM_FP16S : s == "bf16s" ? MPLBM_BF16S : MPLBM_FP32;
+  } else if (token == "math") {
+    file >> s;
+    in.math = s == "fast" ? MPLBM_MATH_FAST : MPLBM_MATH_REFERENCE;
   } else if (token == "kernel") {
     file >> s;
     in.kernel = s == "twopass" ? MPLBM_KERNEL_TWOPASS : s == "fused" ? MPLBM_KERNEL_FUSED : MPLBM_KERNEL_AUTO;

@@ -51,11 +51,23 @@ namespace mplbm {
 // ----------------------------------------------------------------------------
 // storage / arithmetic modes
 
+// kRef: the reference's arithmetic (double-promoted sub-expressions, own
+// exp, no fused multiply-add).  kDev: populations stored as 16-bit
+// deviations.
 struct ModeF32 {
   using S = float;
   using C = float;
   using M = float;  // macro-field type
   static constexpr bool kRef = true;
+  static constexpr bool kDev = false;
+  static constexpr int kStorage = MPLBM_FP32;
+};
+struct ModeF32Fast {  // FP32 storage, FP32 arithmetic with fused multiply-adds
+  using S = float;
+  using C = float;
+  using M = float;
+  static constexpr bool kRef = false;
+  static constexpr bool kDev = false;
   static constexpr int kStorage = MPLBM_FP32;
 };
 struct ModeF64 {
@@ -63,6 +75,7 @@ struct ModeF64 {
   using C = double;
   using M = double;
   static constexpr bool kRef = true;
+  static constexpr bool kDev = false;
   static constexpr int kStorage = MPLBM_FP64;
 };
 struct ModeH16 {
@@ -70,6 +83,7 @@ struct ModeH16 {
   using C = float;
   using M = float;
   static constexpr bool kRef = false;
+  static constexpr bool kDev = true;
   static constexpr int kStorage = MPLBM_FP16S;
 };
 struct ModeB16 {
@@ -77,6 +91,7 @@ struct ModeB16 {
   using C = float;
   using M = float;
   static constexpr bool kRef = false;
+  static constexpr bool kDev = true;
   static constexpr int kStorage = MPLBM_BF16S;
 };
 
@@ -107,22 +122,22 @@ struct Phys {
 
 template <class Mo>
 __device__ __forceinline__ typename Mo::C decode(typename Mo::S s, int i, const Phys<typename Mo::C>& ph) {
-  if constexpr (Mo::kRef) {
+  if constexpr (!Mo::kDev) {
     return s;
   } else {
     float g;
     if constexpr (Mo::kStorage == MPLBM_FP16S) g = __half2float(s);
     else g = __bfloat162float(s);
-    return weight<float>(i) * (g * ph.inv_scale + ph.rho_ref);
+    return weight<float>(i) * __builtin_fmaf(g, ph.inv_scale, ph.rho_ref);
   }
 }
 
 template <class Mo>
 __device__ __forceinline__ typename Mo::S encode(typename Mo::C v, int i, const Phys<typename Mo::C>& ph) {
-  if constexpr (Mo::kRef) {
+  if constexpr (!Mo::kDev) {
     return v;
   } else {
-    const float g = (v * inv_weight_f(i) - ph.rho_ref) * ph.scale;
+    const float g = __builtin_fmaf(v, inv_weight_f(i), -ph.rho_ref) * ph.scale;
     if constexpr (Mo::kStorage == MPLBM_FP16S) return __float2half_rn(g);
     else return __float2bfloat16(g);
   }
@@ -251,16 +266,22 @@ __device__ __forceinline__ void collide(typename Mo::C* f, typename Mo::C rho, t
       f[i] = (C)(fi + ((double)trho * poly - fi) * itau);
     }
   } else {
-    const float inv_rho = 1.0f / rho;
+    // FP32 with fused multiply-adds: feq = w rho (c0 + eu (3 + 4.5 eu)),
+    // c0 = 1 - 1.5 u^2; f += (feq - f) / tau
+    const float inv_rho = __builtin_amdgcn_rcpf(rho);
     ux *= inv_rho;
     uy *= inv_rho;
     uz *= inv_rho;
-    const float usq15 = 1.5f * (ux * ux + uy * uy + uz * uz);
+    const float c0 = __builtin_fmaf(-1.5f, __builtin_fmaf(ux, ux, __builtin_fmaf(uy, uy, uz * uz)), 1.0f);
+    const float rw[3] = {weight<float>(0) * rho, weight<float>(1) * rho, weight<float>(4) * rho};
 #pragma unroll
     for (int i = 0; i < kQ; ++i) {
-      const float eu = (float)ex(i) * ux + (float)ey(i) * uy + (float)ez(i) * uz;
-      const float feq = weight<float>(i) * rho * (1.0f + 3.0f * eu + 4.5f * eu * eu - usq15);
-      f[i] = f[i] + (feq - f[i]) * ph.inv_tau;
+      float eu = 0.f;
+      if (ex(i)) eu = ex(i) > 0 ? eu + ux : eu - ux;
+      if (ey(i)) eu = ey(i) > 0 ? eu + uy : eu - uy;
+      if (ez(i)) eu = ez(i) > 0 ? eu + uz : eu - uz;
+      const float feq = rw[wclass(i)] * __builtin_fmaf(eu, __builtin_fmaf(4.5f, eu, 3.0f), c0);
+      f[i] = __builtin_fmaf(ph.inv_tau, feq - f[i], f[i]);
     }
   }
   u[0] = ux;
@@ -419,27 +440,14 @@ __device__ __forceinline__ void store_cell(uint32_t lk, OwnFn own, UpFn up, cons
 // ----------------------------------------------------------------------------
 // fused marching stream-collide kernel
 
-template <class Mo, bool kFaces, int kBlock>
-__global__ __launch_bounds__(kBlock) void k_fused(Lattice<Mo> L, Phys<typename Mo::C> ph, Tiling T) {
+// One workgroup's share of a step: tile (tyi, tzi) of the (y,z) plane,
+// marched over the x-planes of segment `seg`.  Every thread of the workgroup
+// must call it (it contains barriers); `lds` holds 4 * TY * TZ values of C.
+template <class Mo, bool kFaces>
+__device__ __forceinline__ void march_tile(const Lattice<Mo>& L, const Phys<typename Mo::C>& ph, const Tiling& T,
+                                           int seg, int tyi, int tzi, typename Mo::C* lds) {
   using C = typename Mo::C;
   using S = typename Mo::S;
-  extern __shared__ unsigned char smem_raw[];
-  C* lds = reinterpret_cast<C*>(smem_raw);
-
-  int b = blockIdx.x;
-  if (T.xcd_remap) {
-    // consecutive logical tiles share an XCD (and its L2) so the ring
-    // re-reads of neighbouring tiles hit cache
-    const int per = gridDim.x >> 3;
-    b = (b & 7) * per + (b >> 3);
-  }
-  if (b >= T.nblocks) return;  // whole padding block leaves before any barrier
-  const int ntile = T.tiles_y * T.tiles_z;
-  const int seg = b / ntile;
-  const int t = b - seg * ntile;
-  const int tyi = t / T.tiles_z;
-  const int tzi = t - tyi * T.tiles_z;
-
   const int TY = T.TY, TZ = T.TZ, TT = TY * TZ;
   const int tid = threadIdx.x;
   const bool thr = tid < TT;
@@ -575,6 +583,24 @@ __global__ __launch_bounds__(kBlock) void k_fused(Lattice<Mo> L, Phys<typename M
     rhoP = rhoN;
     lkP = lkN;
   }
+}
+
+template <class Mo, bool kFaces, int kBlock>
+__global__ __launch_bounds__(kBlock) void k_fused(Lattice<Mo> L, Phys<typename Mo::C> ph, Tiling T) {
+  extern __shared__ unsigned char smem_raw[];
+  int b = blockIdx.x;
+  if (T.xcd_remap) {
+    // consecutive logical tiles share an XCD (and its L2) so the ring
+    // re-reads of neighbouring tiles hit cache
+    const int per = gridDim.x >> 3;
+    b = (b & 7) * per + (b >> 3);
+  }
+  if (b >= T.nblocks) return;  // whole padding block leaves before any barrier
+  const int ntile = T.tiles_y * T.tiles_z;
+  const int seg = b / ntile;
+  const int t = b - seg * ntile;
+  const int tyi = t / T.tiles_z;
+  march_tile<Mo, kFaces>(L, ph, T, seg, tyi, t - tyi * T.tiles_z, reinterpret_cast<typename Mo::C*>(smem_raw));
 }
 
 // ----------------------------------------------------------------------------
@@ -1001,7 +1027,9 @@ static mplbm_status launch_step(mplbm_ctx* ctx, int64_t step, int32_t reverse, u
 template <class F>
 static mplbm_status dispatch(mplbm_ctx* ctx, F&& f) {
   switch (ctx->cfg.storage) {
-    case MPLBM_FP32: return f(ModeF32{});
+    case MPLBM_FP32:
+      if (ctx->cfg.math == MPLBM_MATH_FAST) return f(ModeF32Fast{});
+      return f(ModeF32{});
     case MPLBM_FP64: return f(ModeF64{});
     case MPLBM_FP16S: return f(ModeH16{});
     case MPLBM_BF16S: return f(ModeB16{});
@@ -1040,6 +1068,10 @@ mplbm_status mplbm_create(const mplbm_config* cfg, mplbm_ctx** out) {
   if (cfg->storage < MPLBM_FP32 || cfg->storage > MPLBM_BF16S) return fail(nullptr, MPLBM_ERR_ARG, "bad storage");
   if (!(cfg->tau > 0.5)) return fail(nullptr, MPLBM_ERR_ARG, "tau must exceed 0.5");
   if (cfg->carn_star && cfg->G == 0.0) return fail(nullptr, MPLBM_ERR_ARG, "Carnahan-Starling psi needs G != 0");
+  if (cfg->math != MPLBM_MATH_REFERENCE && cfg->math != MPLBM_MATH_FAST)
+    return fail(nullptr, MPLBM_ERR_ARG, "bad math mode");
+  if (cfg->math == MPLBM_MATH_FAST && cfg->storage == MPLBM_FP64)
+    return fail(nullptr, MPLBM_ERR_UNSUPPORTED, "FP64 storage has reference arithmetic only");
   for (int a = 0; a < 3; ++a) {
     if ((cfg->face_type[2 * a] != 0) != (cfg->face_type[2 * a + 1] != 0))
       return fail(nullptr, MPLBM_ERR_ARG, "faces of one axis must both be periodic or both non-periodic");

@@ -35,7 +35,7 @@ class Input(ctypes.Structure):
         ("read_densities", ctypes.c_int32), ("densities_filename", ctypes.c_char * 256),
         ("hysteresis", ctypes.c_int32), ("max_saturation", ctypes.c_double), ("min_saturation", ctypes.c_double),
         ("erosion", ctypes.c_int32), ("target_density", ctypes.c_double),
-        ("storage", ctypes.c_int32), ("kernel", ctypes.c_int32),
+        ("storage", ctypes.c_int32), ("kernel", ctypes.c_int32), ("math", ctypes.c_int32),
         ("face_type", ctypes.c_int32 * 6), ("face_rho", ctypes.c_double * 6),
         ("body_force", ctypes.c_double * 3), ("unknown_tokens", ctypes.c_int32),
     ]

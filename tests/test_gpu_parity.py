@@ -184,3 +184,19 @@ def test_deviation_storage_tracks_fp64(storage):
     err = np.linalg.norm(h_rho[fluid] - st.rho[fluid]) / np.linalg.norm(st.rho[fluid])
     print(f"{storage} density rel-L2 after 50 steps: {err:.3e}", flush=True)
     assert err < (2e-3 if storage == "fp16s" else 2e-2)
+
+
+def test_fp32_fast_math_error_vs_oracle():
+    """FAST arithmetic (FP32 with fused multiply-adds, hardware exp and
+    reciprocal) against the reference-arithmetic oracle on the config-1 case
+    after 100 steps.  The measured error is printed; the bar asserted is the
+    1e-5 relative density tolerance stated for the FP32 form."""
+    shape = (64, 64, 64)
+    fluid, rho = _case(shape, 40, 3.0, 6.0, seed=1)
+    print("case fp32 fast config1", flush=True)
+    st, (g_rho, g_p, g_u, g_f, stats) = _run_pair(shape, fluid, rho, 100, "fp32fast", "fused", **SC)
+    rel_rho = _relerr(g_rho[fluid], st.rho[fluid])
+    umax = float(np.abs(st.u[fluid]).max())
+    rel_u = _abserr(g_u[fluid], st.u[fluid]) / umax
+    print(f"fp32 fast vs oracle after 100 steps: max rel density {rel_rho:.3e}, max |du|/max|u| {rel_u:.3e}")
+    assert rel_rho <= 1e-5
