@@ -99,7 +99,7 @@ typedef struct mplbm_config {
   double dev_scale;            /* FP16S/BF16S scale; 0 = choose automatically */
   int32_t tile_y, tile_z;      /* FUSED: threads per tile in y and z (0 = tuned default) */
   int32_t seg_x;               /* FUSED: planes marched per block (0 = tuned default) */
-  int32_t reserved1;
+  int32_t plane_pad;           /* extra elements per population sub-plane (-1 = automatic) */
 } mplbm_config;
 
 /* FP64 reductions over fluid nodes (mplbm_reduce). */

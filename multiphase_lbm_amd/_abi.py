@@ -35,7 +35,7 @@ class Config(ctypes.Structure):
         ("body_force", ctypes.c_double * 3),
         ("rho_ref", ctypes.c_double), ("dev_scale", ctypes.c_double),
         ("tile_y", ctypes.c_int32), ("tile_z", ctypes.c_int32),
-        ("seg_x", ctypes.c_int32), ("reserved1", ctypes.c_int32),
+        ("seg_x", ctypes.c_int32), ("plane_pad", ctypes.c_int32),
     ]
 
 

@@ -352,10 +352,11 @@ struct Lattice {
   // element index of population i of cell c (plane-major layout)
   __device__ __forceinline__ uint64_t pix(int i, uint32_t c) const {
     const uint32_t x = c / plane;
-    return ((uint64_t)x * kQ + i) * plane + (c - x * plane);
+    return ((uint64_t)x * kQ + i) * ps + (c - x * plane);
   }
   int lx, ly, lz;
-  uint32_t plane;        // ly*lz
+  uint32_t plane;        // ly*lz (cells of an x-plane; field arrays use it)
+  uint32_t ps;           // population sub-plane stride in elements (>= plane)
   uint32_t ncell;
 };
 
@@ -376,8 +377,37 @@ __device__ __forceinline__ bool on_face(const Phys<C>& ph, int x, int y, int z, 
          (ph.face[3] && y == ly - 1) || (ph.face[4] && z == 0) || (ph.face[5] && z == lz - 1);
 }
 
-// Global loads/stores at a 32-bit byte offset from a wave-uniform base: the
-// form that maps to `global_load/store ... v_off, s[base:base+1]`.
+// Raw buffer access: a wave-uniform buffer resource (one x-plane of
+// populations), a per-lane byte offset (voffset) and a wave-uniform byte
+// offset (soffset, the population's sub-plane).  No per-lane address
+// arithmetic.  Word 3 0x00020000 is the gfx9-family resource format.
+using Rsrc = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ Rsrc make_rsrc(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)0xFFFFFFFF, 0x00020000);
+}
+template <class S>
+__device__ __forceinline__ S bload(Rsrc r, uint32_t voff, uint32_t soff) {
+  if constexpr (sizeof(S) == 4) {
+    return __builtin_bit_cast(S, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+  } else if constexpr (sizeof(S) == 8) {
+    return __builtin_bit_cast(S, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+  } else {
+    return __builtin_bit_cast(S, __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0));
+  }
+}
+template <class S>
+__device__ __forceinline__ void bstore(S v, Rsrc r, uint32_t voff, uint32_t soff) {
+  if constexpr (sizeof(S) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)voff, (int)soff, 0);
+  } else if constexpr (sizeof(S) == 8) {
+    using V2 = unsigned int __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, (int)voff, (int)soff, 0);
+  } else {
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, (int)voff, (int)soff, 0);
+  }
+}
+
+// Global loads/stores at a 32-bit byte offset from a base pointer.
 template <class S>
 __device__ __forceinline__ S ld(const S* base, uint32_t off_bytes) {
   return *reinterpret_cast<const S*>(reinterpret_cast<const char*>(base) + off_bytes);
@@ -495,8 +525,16 @@ __device__ __forceinline__ void march_tile(const Lattice<Mo>& L, const Phys<type
   const int lcol[3] = {colm, tz, colp};
 
   const C psi_solid = psi_of<Mo>(ph, ph.rhos);
-  const int x0 = seg * T.seg;
-  const int x1 = min(x0 + T.seg, L.lx);
+  // block-uniform; readfirstlane keeps the march counter and the plane bases
+  // in scalar registers
+  const int x0 = __builtin_amdgcn_readfirstlane(seg * T.seg);
+  const int x1 = __builtin_amdgcn_readfirstlane(min(x0 + T.seg, L.lx));
+
+  // An x-plane of populations is one base pointer (wave-uniform) plus a
+  // 32-bit byte offset i*psb + (y,z) offset: three bases per plane step.
+  const uint32_t psb = L.ps * (uint32_t)sizeof(S);
+  auto in_plane = [&](int xs) { return L.fin + (uint64_t)xs * kQ * L.ps; };
+  auto out_plane = [&](int xs) { return L.fout + (uint64_t)xs * kQ * L.ps; };
 
   // pulls plane xp into f, returns rho (meaningless for solid cells) and link
   auto load_plane = [&](int xp, C* f, uint32_t& lk) -> C {
@@ -506,11 +544,13 @@ __device__ __forceinline__ void march_tile(const Lattice<Mo>& L, const Phys<type
     lk = thr ? L.link[p0 + self_yz] : 1u;
     C r = (C)0;
     if (!(lk & 1u)) {
+      const Rsrc bm = make_rsrc(in_plane(xm));
+      const Rsrc b0 = make_rsrc(in_plane(xp));
+      const Rsrc bq = make_rsrc(in_plane(xq));
 #pragma unroll
       for (int i = 0; i < kQ; ++i) {
-        const uint64_t xs = ex(i) == -1 ? xq : (ex(i) == 1 ? xm : xp);
-        const S* base = L.fin + (xs * kQ + i) * L.plane;
-        f[i] = decode<Mo>(ld(base, yzb[1 - ey(i)][1 - ez(i)]), i, ph);
+        const Rsrc base = ex(i) == -1 ? bq : (ex(i) == 1 ? bm : b0);
+        f[i] = decode<Mo>(bload<S>(base, yzb[1 - ey(i)][1 - ez(i)], (uint32_t)i * psb), i, ph);
       }
       if constexpr (kFaces) {
         if (on_face(ph, xp, gy, gz, L.lx, L.ly, L.lz)) apply_faces<Mo>(L, ph, (uint32_t)p0 + self_yz, xp, gy, gz, f);
@@ -534,7 +574,7 @@ __device__ __forceinline__ void march_tile(const Lattice<Mo>& L, const Phys<type
   }
 
   for (int j = 0; j < x1 - x0; ++j) {
-    const int x = x0 + j;
+    const int x = __builtin_amdgcn_readfirstlane(x0 + j);
     const int xn = x + 1 == L.lx ? 0 : x + 1;
     rhoN = load_plane(xn, N, lkN);
     if (thr) lds[((j + 2) & 3) * TT + myl] = (lkN & 1u) ? psi_solid : psi_of<Mo>(ph, rhoN);
@@ -560,15 +600,20 @@ __device__ __forceinline__ void march_tile(const Lattice<Mo>& L, const Phys<type
       }
       C u[3];
       collide<Mo>(P, rhoP, F, ph, u);
-      store_cell<Mo>(lkP,
-                     [&](int i) {
-                       return PtrOff<S*, uint32_t>{L.fout + ((uint64_t)x * kQ + i) * L.plane, yzb[1][1]};
-                     },
-                     [&](int i) {
-                       const uint64_t xs = ex(i) == -1 ? (uint64_t)xn : (ex(i) == 1 ? xm : (uint64_t)x);
-                       return PtrOff<S*, uint32_t>{L.fout + (xs * kQ + i) * L.plane, yzb[1 - ey(i)][1 - ez(i)]};
-                     },
-                     P, ph);
+      const Rsrc om = make_rsrc(out_plane((int)xm));
+      const Rsrc o0 = make_rsrc(out_plane(x));
+      const Rsrc oq = make_rsrc(out_plane(xn));
+#pragma unroll
+      for (int i = 0; i < kQ; ++i) bstore<S>(encode<Mo>(P[i], i, ph), o0, yzb[1][1], (uint32_t)i * psb);
+      if (lkP & ~1u) {
+        // write-side bounce-back into the solid upstream nodes (see store_cell)
+#pragma unroll
+        for (int i = 1; i < kQ; ++i)
+          if ((lkP >> i) & 1u) {
+            const Rsrc base = ex(i) == -1 ? oq : (ex(i) == 1 ? om : o0);
+            bstore<S>(encode<Mo>(P[opp(i)], i, ph), base, yzb[1 - ey(i)][1 - ez(i)], (uint32_t)i * psb);
+          }
+      }
       if (ph.write_macro) {
         const uint64_t self = p0 + self_yz;
         L.rho[self] = (typename Mo::M)rhoP;
@@ -844,6 +889,7 @@ struct mplbm_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   uint64_t ncell = 0;
+  uint64_t ps = 0;  // population sub-plane stride (elements)
   int bytes_pop = 4;
   void* lat[2] = {nullptr, nullptr};  // population lattices
   int cur = 0;                        // lattice holding the newest post-collision state
@@ -942,6 +988,7 @@ static Lattice<Mo> make_lattice(const mplbm_ctx* ctx, int in, int out) {
   L.ly = (int)ctx->cfg.ly;
   L.lz = (int)ctx->cfg.lz;
   L.plane = (uint32_t)(ctx->cfg.ly * ctx->cfg.lz);
+  L.ps = (uint32_t)ctx->ps;
   L.ncell = (uint32_t)ctx->ncell;
   return L;
 }
@@ -1055,6 +1102,7 @@ void mplbm_config_default(mplbm_config* c) {
   c->rho_increment = 0.0;
   c->rhos = 0.0;
   c->inject_position = 1.0;
+  c->plane_pad = -1;
 }
 
 const char* mplbm_last_error(const mplbm_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
@@ -1084,6 +1132,17 @@ mplbm_status mplbm_create(const mplbm_config* cfg, mplbm_ctx** out) {
   ctx->ncell = n;
   ctx->bytes_pop = (int)storage_bytes(cfg->storage);
   ctx->kernel = cfg->kernel == MPLBM_KERNEL_TWOPASS ? MPLBM_KERNEL_TWOPASS : MPLBM_KERNEL_FUSED;
+  {
+    // sub-plane stride: ly*lz rounded up to 64 elements plus plane_pad
+    // elements; a negative plane_pad picks 64 elements when the rounded plane
+    // is a multiple of 4096 elements (power-of-two sizes map every
+    // sub-plane of a cell to the same memory channel otherwise)
+    const uint64_t plane = (uint64_t)cfg->ly * cfg->lz;
+    uint64_t ps = (plane + 63) / 64 * 64;
+    int64_t pad = cfg->plane_pad;
+    if (pad < 0) pad = (ps % 4096 == 0) ? 64 : 0;
+    ctx->ps = ps + (uint64_t)pad;
+  }
   mplbm_status st = choose_tiling(ctx);
   if (st != MPLBM_OK) {
     g_create_error = ctx->err;
@@ -1106,7 +1165,7 @@ mplbm_status mplbm_create(const mplbm_config* cfg, mplbm_ctx** out) {
   CREATE_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   CREATE_TRY(hipEventCreate(&ctx->ev0));
   CREATE_TRY(hipEventCreate(&ctx->ev1));
-  const size_t lat_bytes = (size_t)n * kQ * ctx->bytes_pop;
+  const size_t lat_bytes = (size_t)cfg->lx * kQ * ctx->ps * ctx->bytes_pop;
   const size_t mb = cfg->storage == MPLBM_FP64 ? 8 : 4;  // macro / psi element size
   CREATE_TRY(hipMalloc(&ctx->lat[0], lat_bytes));
   CREATE_TRY(hipMalloc(&ctx->lat[1], lat_bytes));
