@@ -536,53 +536,80 @@ __device__ __forceinline__ void march_tile(const Lattice<Mo>& L, const Phys<type
   auto in_plane = [&](int xs) { return L.fin + (uint64_t)xs * kQ * L.ps; };
   auto out_plane = [&](int xs) { return L.fout + (uint64_t)xs * kQ * L.ps; };
 
-  // pulls plane xp into f, returns rho (meaningless for solid cells) and link
-  auto load_plane = [&](int xp, C* f, uint32_t& lk) -> C {
-    const int xm = xp == 0 ? L.lx - 1 : xp - 1;
-    const int xq = xp + 1 == L.lx ? 0 : xp + 1;
-    const uint64_t p0 = (uint64_t)xp * L.plane;
-    lk = thr ? L.link[p0 + self_yz] : 1u;
-    C r = (C)0;
+  auto wrapx = [&](int v) { return v >= L.lx ? v - L.lx : (v < 0 ? v + L.lx : v); };
+  auto link_of = [&](int xp) -> uint32_t { return thr ? L.link[(uint64_t)xp * L.plane + self_yz] : 1u; };
+
+  // Issue the 19 pulls of plane xp (fluid cells only) into raw storage; the
+  // values are consumed one pipeline stage later.
+  auto issue_plane = [&](int xp, uint32_t lk, S* raw) {
     if (!(lk & 1u)) {
-      const Rsrc bm = make_rsrc(in_plane(xm));
+      const Rsrc bm = make_rsrc(in_plane(wrapx(xp - 1)));
       const Rsrc b0 = make_rsrc(in_plane(xp));
-      const Rsrc bq = make_rsrc(in_plane(xq));
+      const Rsrc bq = make_rsrc(in_plane(wrapx(xp + 1)));
 #pragma unroll
       for (int i = 0; i < kQ; ++i) {
         const Rsrc base = ex(i) == -1 ? bq : (ex(i) == 1 ? bm : b0);
-        f[i] = decode<Mo>(bload<S>(base, yzb[1 - ey(i)][1 - ez(i)], (uint32_t)i * psb), i, ph);
+        raw[i] = bload<S>(base, yzb[1 - ey(i)][1 - ez(i)], (uint32_t)i * psb);
       }
+    }
+  };
+  // Decode the pulled populations, apply face rules, return rho.
+  auto finish_plane = [&](int xp, uint32_t lk, const S* raw, C* f) -> C {
+    C r = (C)0;
+    if (!(lk & 1u)) {
+#pragma unroll
+      for (int i = 0; i < kQ; ++i) f[i] = decode<Mo>(raw[i], i, ph);
       if constexpr (kFaces) {
-        if (on_face(ph, xp, gy, gz, L.lx, L.ly, L.lz)) apply_faces<Mo>(L, ph, (uint32_t)p0 + self_yz, xp, gy, gz, f);
+        if (on_face(ph, xp, gy, gz, L.lx, L.ly, L.lz))
+          apply_faces<Mo>(L, ph, (uint32_t)((uint64_t)xp * L.plane) + self_yz, xp, gy, gz, f);
       }
       r = density(f, gy, ph);
     }
     return r;
   };
-
-  C P[kQ], N[kQ];
-  C rhoP, rhoN;
-  uint32_t lkP, lkN;
   const int myl = ty * TZ + tz;
+  auto put_psi = [&](int slot, uint32_t lk, C rho) {
+    if (thr) lds[slot * TT + myl] = (lk & 1u) ? psi_solid : psi_of<Mo>(ph, rho);
+  };
 
-  {  // warm-up: plane x0-1 (psi only) and plane x0 (kept)
-    const int xw = x0 == 0 ? L.lx - 1 : x0 - 1;
-    const C r = load_plane(xw, N, lkN);
-    if (thr) lds[0 * TT + myl] = (lkN & 1u) ? psi_solid : psi_of<Mo>(ph, r);
-    rhoP = load_plane(x0, P, lkP);
-    if (thr) lds[1 * TT + myl] = (lkP & 1u) ? psi_solid : psi_of<Mo>(ph, rhoP);
+  // Three-stage pipeline over the planes of the segment: while plane x is
+  // collided and plane x+1's density/psi is formed, the loads of plane x+2
+  // and the link word of plane x+3 are in flight.
+  const int n = x1 - x0;
+  S rawQ[kQ], rawN[kQ];
+  C P[kQ], Qd[kQ];
+  C rhoP;
+  uint32_t lkP, lkQ, lkN;
+  {
+    const int xw = wrapx(x0 - 1);
+    const uint32_t lkW = link_of(xw);
+    issue_plane(xw, lkW, rawN);
+    lkP = link_of(x0);
+    issue_plane(x0, lkP, rawQ);
+    const C rw = finish_plane(xw, lkW, rawN, Qd);
+    put_psi(0, lkW, rw);
+    rhoP = finish_plane(x0, lkP, rawQ, P);
+    put_psi(1, lkP, rhoP);
+    lkQ = link_of(wrapx(x0 + 1));
+    issue_plane(wrapx(x0 + 1), lkQ, rawQ);
+    lkN = n > 1 ? link_of(wrapx(x0 + 2)) : 1u;
   }
 
-  for (int j = 0; j < x1 - x0; ++j) {
+  for (int j = 0; j < n; ++j) {
     const int x = __builtin_amdgcn_readfirstlane(x0 + j);
-    const int xn = x + 1 == L.lx ? 0 : x + 1;
-    rhoN = load_plane(xn, N, lkN);
-    if (thr) lds[((j + 2) & 3) * TT + myl] = (lkN & 1u) ? psi_solid : psi_of<Mo>(ph, rhoN);
+    const int xn = wrapx(x + 1);
+    uint32_t lkNext = 1u;
+    if (j + 1 < n) {  // uniform: plane x+2 feeds the next iteration
+      issue_plane(wrapx(x + 2), lkN, rawN);
+      if (j + 2 < n) lkNext = link_of(wrapx(x + 3));
+    }
+    const C rhoQ = finish_plane(xn, lkQ, rawQ, Qd);
+    put_psi((j + 2) & 3, lkQ, rhoQ);
     __syncthreads();
 
     if (interior && !(lkP & 1u)) {
       const uint64_t p0 = (uint64_t)x * L.plane;
-      const uint64_t xm = (uint64_t)(x == 0 ? L.lx - 1 : x - 1);
+      const int xm = wrapx(x - 1);
       C F[3];
       bool skip = false;
       if constexpr (kFaces) skip = force_skipped(ph, x, gy, gz, L.lx, L.ly, L.lz);
@@ -600,7 +627,7 @@ __device__ __forceinline__ void march_tile(const Lattice<Mo>& L, const Phys<type
       }
       C u[3];
       collide<Mo>(P, rhoP, F, ph, u);
-      const Rsrc om = make_rsrc(out_plane((int)xm));
+      const Rsrc om = make_rsrc(out_plane(xm));
       const Rsrc o0 = make_rsrc(out_plane(x));
       const Rsrc oq = make_rsrc(out_plane(xn));
 #pragma unroll
@@ -624,9 +651,14 @@ __device__ __forceinline__ void march_tile(const Lattice<Mo>& L, const Phys<type
       }
     }
 #pragma unroll
-    for (int i = 0; i < kQ; ++i) P[i] = N[i];
-    rhoP = rhoN;
-    lkP = lkN;
+    for (int i = 0; i < kQ; ++i) {
+      P[i] = Qd[i];
+      rawQ[i] = rawN[i];
+    }
+    rhoP = rhoQ;
+    lkP = lkQ;
+    lkQ = lkN;
+    lkN = lkNext;
   }
 }
 
@@ -1577,6 +1609,387 @@ void mplbm_compat_release(void) {
   if (g_compat.ctx) mplbm_destroy(g_compat.ctx);
   g_compat.ctx = nullptr;
   g_compat.rho = g_compat.pres = nullptr;
+}
+
+}  // extern "C"
+
+// ============================================================================
+// batched ensemble (include/mplbm_ensemble.h)
+
+#include "mplbm_ensemble.h"
+
+namespace mplbm {
+
+struct EnsItem {
+  int sample, seg, tyi, tzi;
+};
+
+template <class Mo>
+struct EnsSample {
+  Lattice<Mo> L[2];  // [parity]: L[0] reads the lattice current at table build
+  Phys<typename Mo::C> ph;
+  Tiling T;
+};
+
+// Persistent launch over the flattened work list; every workgroup walks
+// items blockIdx.x, blockIdx.x + gridDim.x, ...  Sample descriptors are
+// read with wave-uniform addresses (scalar loads).
+template <class Mo, int kBlock>
+__global__ __launch_bounds__(kBlock) void k_ensemble(const EnsItem* __restrict__ items, int nitems,
+                                                     const EnsSample<Mo>* __restrict__ samples, int parity,
+                                                     int macro) {
+  extern __shared__ unsigned char smem_raw[];
+  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    const EnsItem it = items[w];
+    const int s = __builtin_amdgcn_readfirstlane(it.sample);
+    const EnsSample<Mo>& sd = samples[s];
+    Phys<typename Mo::C> ph = sd.ph;
+    ph.write_macro = macro;
+    march_tile<Mo, false>(sd.L[parity], ph, sd.T, __builtin_amdgcn_readfirstlane(it.seg),
+                          __builtin_amdgcn_readfirstlane(it.tyi), __builtin_amdgcn_readfirstlane(it.tzi),
+                          reinterpret_cast<typename Mo::C*>(smem_raw));
+    __syncthreads();  // the next item reuses the LDS ring
+  }
+}
+
+}  // namespace mplbm
+
+struct mplbm_ensemble {
+  std::vector<mplbm_ctx*> ctx;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int block = 512;
+  int storage = MPLBM_FP32, math = MPLBM_MATH_REFERENCE;
+  std::vector<EnsItem> items;
+  EnsItem* d_items = nullptr;
+  void* d_samples = nullptr;
+  size_t sample_bytes = 0;
+  size_t lds_bytes = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int64_t fluid_nodes = 0;
+  std::string err;
+};
+
+#define ENS_HIP(e, call)                                                                          \
+  do {                                                                                            \
+    hipError_t e_ = (call);                                                                       \
+    if (e_ != hipSuccess)                                                                         \
+      return ens_fail((e), e_ == hipErrorOutOfMemory ? MPLBM_ERR_NOMEM : MPLBM_ERR_HIP, "%s: %s", #call, \
+                      hipGetErrorString(e_));                                                     \
+  } while (0)
+
+static mplbm_status ens_fail(mplbm_ensemble* e, mplbm_status st, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (e) e->err = buf;
+  else g_create_error = buf;
+  return st;
+}
+
+// Tile of one sample for a workgroup of `block` threads: an axis that fits
+// wraps inside the tile, otherwise a one-cell ring.
+static void ens_fit_tile(const mplbm_config& c, int block, int* ty, int* tz, int* seg) {
+  int z = c.lz <= 64 ? (int)c.lz : (block >= 512 ? 66 : 34);
+  if (z > block / 3) z = 34;
+  int y = block / z;
+  if (y >= c.ly && (int64_t)z * c.ly <= block) y = (int)c.ly;
+  if (y > c.ly) y = (int)c.ly;
+  *ty = y;
+  *tz = z;
+  *seg = (int)std::min<int64_t>(32, c.lx);
+}
+
+static mplbm_status ens_plan(mplbm_ensemble* e) {
+  e->items.clear();
+  e->lds_bytes = 0;
+  const size_t csize = (e->storage == MPLBM_FP64) ? 8 : 4;
+  for (size_t s = 0; s < e->ctx.size(); ++s) {
+    mplbm_ctx* c = e->ctx[s];
+    int ty, tz, seg;
+    ens_fit_tile(c->cfg, e->block, &ty, &tz, &seg);
+    mplbm_status st = mplbm_set_tiling(c, ty, tz, seg);
+    if (st != MPLBM_OK) return ens_fail(e, st, "sample %zu: %s", s, c->err.c_str());
+    const Tiling& T = c->tiling;
+    if (T.TY * T.TZ > e->block)
+      return ens_fail(e, MPLBM_ERR_ARG, "sample %zu: tile %dx%d exceeds %d threads", s, T.TY, T.TZ, e->block);
+    e->lds_bytes = std::max(e->lds_bytes, 4 * (size_t)T.TY * T.TZ * csize);
+  }
+  // items ordered by segment, then sample, then tile: consecutive workgroups
+  // march neighbouring tiles of one sample, so ring re-reads meet in L2
+  int max_seg = 0;
+  for (mplbm_ctx* c : e->ctx) max_seg = std::max(max_seg, c->tiling.nseg);
+  for (int g = 0; g < max_seg; ++g)
+    for (size_t s = 0; s < e->ctx.size(); ++s) {
+      const Tiling& T = e->ctx[s]->tiling;
+      if (g >= T.nseg) continue;
+      for (int a = 0; a < T.tiles_y; ++a)
+        for (int b = 0; b < T.tiles_z; ++b) e->items.push_back(EnsItem{(int)s, g, a, b});
+    }
+  if (e->d_items) (void)hipFree(e->d_items);
+  e->d_items = nullptr;
+  ENS_HIP(e, hipMalloc((void**)&e->d_items, e->items.size() * sizeof(EnsItem)));
+  ENS_HIP(e, hipMemcpy(e->d_items, e->items.data(), e->items.size() * sizeof(EnsItem), hipMemcpyHostToDevice));
+  if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  e->graph_exec = nullptr;
+  e->graph = nullptr;
+  return MPLBM_OK;
+}
+
+template <class F>
+static mplbm_status ens_dispatch(mplbm_ensemble* e, F&& f) {
+  switch (e->storage) {
+    case MPLBM_FP32:
+      if (e->math == MPLBM_MATH_FAST) return f(ModeF32Fast{});
+      return f(ModeF32{});
+    case MPLBM_FP64: return f(ModeF64{});
+    case MPLBM_FP16S: return f(ModeH16{});
+    case MPLBM_BF16S: return f(ModeB16{});
+  }
+  return ens_fail(e, MPLBM_ERR_ARG, "bad storage");
+}
+
+// Upload the per-sample descriptor table for the lattices current now;
+// inject_now follows `step`.
+static mplbm_status ens_upload(mplbm_ensemble* e, int64_t step) {
+  return ens_dispatch(e, [&](auto mo) -> mplbm_status {
+    using Mo = decltype(mo);
+    std::vector<EnsSample<Mo>> tab(e->ctx.size());
+    for (size_t s = 0; s < e->ctx.size(); ++s) {
+      mplbm_ctx* c = e->ctx[s];
+      tab[s].L[0] = make_lattice<Mo>(c, c->cur, 1 - c->cur);
+      tab[s].L[1] = make_lattice<Mo>(c, 1 - c->cur, c->cur);
+      tab[s].ph = make_phys<typename Mo::C>(c, step, 1, 0, false);
+      tab[s].T = c->tiling;
+    }
+    const size_t bytes = tab.size() * sizeof(EnsSample<Mo>);
+    if (e->sample_bytes != bytes) {
+      if (e->d_samples) (void)hipFree(e->d_samples);
+      e->d_samples = nullptr;
+      ENS_HIP(e, hipMalloc(&e->d_samples, bytes));
+      e->sample_bytes = bytes;
+      if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+      if (e->graph) (void)hipGraphDestroy(e->graph);
+      e->graph_exec = nullptr;
+      e->graph = nullptr;
+    }
+    // stream-ordered after earlier steps; synchronous so `tab` can go away
+    ENS_HIP(e, hipStreamSynchronize(e->stream));
+    ENS_HIP(e, hipMemcpy(e->d_samples, tab.data(), bytes, hipMemcpyHostToDevice));
+    return MPLBM_OK;
+  });
+}
+
+static mplbm_status ens_launch(mplbm_ensemble* e, int parity, int macro) {
+  return ens_dispatch(e, [&](auto mo) -> mplbm_status {
+    using Mo = decltype(mo);
+    int dev = 0, ncu = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const int nitems = (int)e->items.size();
+    const int per_cu = e->block >= 1024 ? 2 : (e->block >= 512 ? 3 : 6);
+    const int grid = std::max(1, std::min(nitems, ncu * per_cu));
+    const auto* samp = static_cast<const EnsSample<Mo>*>(e->d_samples);
+    if (e->block == 1024)
+      hipLaunchKernelGGL((k_ensemble<Mo, 1024>), dim3(grid), dim3(1024), e->lds_bytes, e->stream, e->d_items,
+                         nitems, samp, parity, macro);
+    else if (e->block == 512)
+      hipLaunchKernelGGL((k_ensemble<Mo, 512>), dim3(grid), dim3(512), e->lds_bytes, e->stream, e->d_items, nitems,
+                         samp, parity, macro);
+    else
+      hipLaunchKernelGGL((k_ensemble<Mo, 256>), dim3(grid), dim3(256), e->lds_bytes, e->stream, e->d_items, nitems,
+                         samp, parity, macro);
+    ENS_HIP(e, hipGetLastError());
+    return MPLBM_OK;
+  });
+}
+
+extern "C" {
+
+mplbm_status mplbm_ensemble_create(const mplbm_config* cfgs, int32_t n, int32_t block_threads,
+                                   mplbm_ensemble** out) {
+  if (!cfgs || !out || n <= 0) return ens_fail(nullptr, MPLBM_ERR_ARG, "bad arguments");
+  *out = nullptr;
+  if (block_threads == 0) block_threads = 512;
+  if (block_threads != 256 && block_threads != 512 && block_threads != 1024)
+    return ens_fail(nullptr, MPLBM_ERR_ARG, "block_threads must be 256, 512 or 1024");
+  for (int32_t s = 0; s < n; ++s) {
+    if (cfgs[s].storage != cfgs[0].storage || cfgs[s].math != cfgs[0].math)
+      return ens_fail(nullptr, MPLBM_ERR_ARG, "samples must share storage and arithmetic");
+    for (int f = 0; f < 6; ++f)
+      if (cfgs[s].face_type[f] != MPLBM_FACE_PERIODIC)
+        return ens_fail(nullptr, MPLBM_ERR_UNSUPPORTED, "ensemble samples are periodic");
+  }
+  mplbm_ensemble* e = new mplbm_ensemble();
+  e->block = block_threads;
+  e->storage = cfgs[0].storage;
+  e->math = cfgs[0].math;
+  for (int32_t s = 0; s < n; ++s) {
+    mplbm_config c = cfgs[s];
+    c.kernel = MPLBM_KERNEL_FUSED;
+    mplbm_ctx* ctx = nullptr;
+    mplbm_status st = mplbm_create(&c, &ctx);
+    if (st != MPLBM_OK) {
+      const std::string msg = g_create_error;
+      mplbm_ensemble_destroy(e);
+      return ens_fail(nullptr, st, "sample %d: %s", s, msg.c_str());
+    }
+    e->ctx.push_back(ctx);
+  }
+  hipError_t he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+  if (he == hipSuccess) he = hipEventCreate(&e->ev0);
+  if (he == hipSuccess) he = hipEventCreate(&e->ev1);
+  if (he != hipSuccess) {
+    mplbm_ensemble_destroy(e);
+    return ens_fail(nullptr, MPLBM_ERR_HIP, "stream/event: %s", hipGetErrorString(he));
+  }
+  mplbm_status st = ens_plan(e);
+  if (st != MPLBM_OK) {
+    const std::string msg = e->err.empty() ? g_create_error : e->err;
+    mplbm_ensemble_destroy(e);
+    return ens_fail(nullptr, st, "%s", msg.c_str());
+  }
+  *out = e;
+  return MPLBM_OK;
+}
+
+void mplbm_ensemble_destroy(mplbm_ensemble* e) {
+  if (!e) return;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  if (e->d_items) (void)hipFree(e->d_items);
+  if (e->d_samples) (void)hipFree(e->d_samples);
+  for (mplbm_ctx* c : e->ctx) mplbm_destroy(c);
+  if (e->ev0) (void)hipEventDestroy(e->ev0);
+  if (e->ev1) (void)hipEventDestroy(e->ev1);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+const char* mplbm_ensemble_last_error(const mplbm_ensemble* e) { return e ? e->err.c_str() : g_create_error.c_str(); }
+
+#define ENS_SAMPLE(e, s)                                                                  \
+  do {                                                                                    \
+    if (!(e)) return ens_fail(nullptr, MPLBM_ERR_ARG, "null ensemble");                   \
+    if ((s) < 0 || (size_t)(s) >= (e)->ctx.size()) return ens_fail((e), MPLBM_ERR_ARG, "bad sample %d", (int)(s)); \
+  } while (0)
+
+mplbm_status mplbm_ensemble_set_geometry(mplbm_ensemble* e, int32_t s, const uint8_t* is_fluid) {
+  ENS_SAMPLE(e, s);
+  if (hipStreamSynchronize(e->stream) != hipSuccess) return ens_fail(e, MPLBM_ERR_HIP, "sync");
+  mplbm_status st = mplbm_set_geometry(e->ctx[s], is_fluid);
+  if (st != MPLBM_OK) return ens_fail(e, st, "%s", e->ctx[s]->err.c_str());
+  e->fluid_nodes = 0;
+  for (mplbm_ctx* c : e->ctx) e->fluid_nodes += c->fluid_nodes;
+  return MPLBM_OK;
+}
+
+mplbm_status mplbm_ensemble_init_fields(mplbm_ensemble* e, int32_t s, const float* rho) {
+  ENS_SAMPLE(e, s);
+  if (hipStreamSynchronize(e->stream) != hipSuccess) return ens_fail(e, MPLBM_ERR_HIP, "sync");
+  mplbm_status st = mplbm_init_fields(e->ctx[s], rho);
+  if (st != MPLBM_OK) return ens_fail(e, st, "%s", e->ctx[s]->err.c_str());
+  return MPLBM_OK;
+}
+
+mplbm_status mplbm_ensemble_step(mplbm_ensemble* e, int64_t step, int32_t nsteps, uint32_t flags,
+                                 int32_t use_graph) {
+  if (!e) return ens_fail(nullptr, MPLBM_ERR_ARG, "null ensemble");
+  if (nsteps <= 0) return MPLBM_OK;
+  for (mplbm_ctx* c : e->ctx)
+    if (!c->have_fields) return ens_fail(e, MPLBM_ERR_STATE, "every sample needs geometry and fields first");
+  bool injects = false;
+  for (mplbm_ctx* c : e->ctx) injects |= c->cfg.rho_inject_period > 0 && c->cfg.rho_increment != 0.0;
+  const bool macro = (flags & MPLBM_STEP_WRITE_MACRO) != 0;
+  mplbm_status st;
+  if (injects) {
+    // the descriptor table changes with step % period: direct launches
+    for (int32_t k = 0; k < nsteps; ++k) {
+      if ((st = ens_upload(e, step + k)) != MPLBM_OK) return st;
+      if ((st = ens_launch(e, 0, macro && k == nsteps - 1)) != MPLBM_OK) return st;
+      for (mplbm_ctx* c : e->ctx) c->cur = 1 - c->cur;
+    }
+    return MPLBM_OK;
+  }
+  if ((st = ens_upload(e, step)) != MPLBM_OK) return st;
+  const int32_t plain = macro ? nsteps - 1 : nsteps;
+  int parity = 0;
+  int32_t done = 0;
+  if (use_graph && plain >= 2) {
+    if (!e->graph_exec) {
+      ENS_HIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+      mplbm_status s0 = ens_launch(e, 0, 0);
+      mplbm_status s1 = s0 == MPLBM_OK ? ens_launch(e, 1, 0) : s0;
+      hipError_t ce = hipStreamEndCapture(e->stream, &e->graph);
+      if (s1 != MPLBM_OK) return s1;
+      if (ce != hipSuccess) return ens_fail(e, MPLBM_ERR_HIP, "graph capture: %s", hipGetErrorString(ce));
+      ENS_HIP(e, hipGraphInstantiate(&e->graph_exec, e->graph, nullptr, nullptr, 0));
+    }
+    for (; done + 2 <= plain; done += 2) ENS_HIP(e, hipGraphLaunch(e->graph_exec, e->stream));
+  }
+  for (; done < plain; ++done) {
+    if ((st = ens_launch(e, parity ^ (done & 1), 0)) != MPLBM_OK) return st;
+  }
+  if (macro && (st = ens_launch(e, plain & 1, 1)) != MPLBM_OK) return st;
+  if (nsteps & 1)
+    for (mplbm_ctx* c : e->ctx) c->cur = 1 - c->cur;
+  return MPLBM_OK;
+}
+
+mplbm_status mplbm_ensemble_step_timed(mplbm_ensemble* e, int64_t step, int32_t nsteps, uint32_t flags,
+                                       int32_t use_graph, double* elapsed_ms) {
+  if (!e) return ens_fail(nullptr, MPLBM_ERR_ARG, "null ensemble");
+  // build tables and graph first so the timed region holds only the steps
+  ENS_HIP(e, hipStreamSynchronize(e->stream));
+  ENS_HIP(e, hipEventRecord(e->ev0, e->stream));
+  mplbm_status st = mplbm_ensemble_step(e, step, nsteps, flags, use_graph);
+  if (st != MPLBM_OK) return st;
+  ENS_HIP(e, hipEventRecord(e->ev1, e->stream));
+  ENS_HIP(e, hipEventSynchronize(e->ev1));
+  float ms = 0.f;
+  ENS_HIP(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));
+  if (elapsed_ms) *elapsed_ms = ms;
+  return MPLBM_OK;
+}
+
+mplbm_status mplbm_ensemble_read_fields(mplbm_ensemble* e, int32_t s, float* rho, float* pressure, float* u) {
+  ENS_SAMPLE(e, s);
+  if (hipStreamSynchronize(e->stream) != hipSuccess) return ens_fail(e, MPLBM_ERR_HIP, "sync");
+  mplbm_status st = mplbm_read_fields(e->ctx[s], rho, pressure, u);
+  return st == MPLBM_OK ? st : ens_fail(e, st, "%s", e->ctx[s]->err.c_str());
+}
+
+mplbm_status mplbm_ensemble_read_populations_f64(mplbm_ensemble* e, int32_t s, double* f) {
+  ENS_SAMPLE(e, s);
+  if (hipStreamSynchronize(e->stream) != hipSuccess) return ens_fail(e, MPLBM_ERR_HIP, "sync");
+  mplbm_status st = mplbm_read_populations_f64(e->ctx[s], f);
+  return st == MPLBM_OK ? st : ens_fail(e, st, "%s", e->ctx[s]->err.c_str());
+}
+
+mplbm_status mplbm_ensemble_reduce(mplbm_ensemble* e, int32_t s, mplbm_stats64* out) {
+  ENS_SAMPLE(e, s);
+  if (hipStreamSynchronize(e->stream) != hipSuccess) return ens_fail(e, MPLBM_ERR_HIP, "sync");
+  mplbm_status st = mplbm_reduce(e->ctx[s], out);
+  return st == MPLBM_OK ? st : ens_fail(e, st, "%s", e->ctx[s]->err.c_str());
+}
+
+mplbm_status mplbm_ensemble_info(const mplbm_ensemble* e, int32_t s, mplbm_info* out) {
+  if (!e || s < 0 || (size_t)s >= e->ctx.size()) return MPLBM_ERR_ARG;
+  return mplbm_info_get(e->ctx[s], out);
+}
+
+mplbm_status mplbm_ensemble_summary(const mplbm_ensemble* e, int64_t* work_items, int64_t* fluid_nodes,
+                                    int32_t* block_threads) {
+  if (!e) return MPLBM_ERR_ARG;
+  if (work_items) *work_items = (int64_t)e->items.size();
+  if (fluid_nodes) *fluid_nodes = e->fluid_nodes;
+  if (block_threads) *block_threads = e->block;
+  return MPLBM_OK;
 }
 
 }  // extern "C"

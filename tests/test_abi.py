@@ -23,7 +23,8 @@ def _exported(lib):
 
 
 @pytest.mark.parametrize("header,lib", [("mplbm.h", "libmplbm.so"), ("mplbm_host.h", "libmplbm_host.so"),
-                                        ("iteration_host_compat.h", "libmplbm.so")])
+                                        ("iteration_host_compat.h", "libmplbm.so"),
+                                        ("mplbm_ensemble.h", "libmplbm.so")])
 def test_library_exports_every_declared_symbol(header, lib):
     path = os.path.join(ROOT, "multiphase_lbm_amd", lib)
     assert os.path.exists(path), f"{lib} not built"
@@ -40,6 +41,8 @@ def test_python_binding_covers_the_abi():
     from multiphase_lbm_amd import _abi, harness
     assert set(_abi.SIGNATURES) == set(_declared("mplbm.h"))
     assert set(harness.SIGNATURES) == set(_declared("mplbm_host.h"))
+    from multiphase_lbm_amd import ensemble
+    assert set(ensemble.SIGNATURES) == set(_declared("mplbm_ensemble.h")) - set(_declared("mplbm.h"))
 
 
 def test_config_layout_matches_c():
