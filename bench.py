@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--tile-z", type=int, default=0)
     ap.add_argument("--seg-x", type=int, default=0)
+    ap.add_argument("--bounce-back", type=int, default=0, help="0: write-side (ghost) stores, 1: pull-side reads")
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline (0: skip)")
     ap.add_argument("--profile-only", action="store_true", help="run only the FP32 timed loop (for rocprofv3)")
     ap.add_argument("--precision", action="store_true", help="error study of FP32/FP16S/BF16S against FP64")
@@ -76,7 +77,7 @@ PHYS = dict(tau=1.0, G=-5.0, rhos=0.7, rhol_spinodal=1.285931, rhog_spinodal=0.3
 def run_gpu(a, shape, fluid, rho, storage):
     import multiphase_lbm_amd as mp
     ctx = mp.Context(*shape, storage=storage, kernel=a.kernel, tile_y=a.tile_y, tile_z=a.tile_z, seg_x=a.seg_x,
-                     **PHYS)
+                     bounce_back=a.bounce_back, **PHYS)
     try:
         ctx.set_geometry(fluid)
         ctx.init_fields(rho)
@@ -261,7 +262,7 @@ def main():
         import multiphase_lbm_amd as mp
         from multiphase_lbm_amd.autotune import autotune
         for storage in a.storage.split(","):
-            with mp.Context(*shape, storage=storage, kernel="fused", **PHYS) as ctx:
+            with mp.Context(*shape, storage=storage, kernel="fused", bounce_back=a.bounce_back, **PHYS) as ctx:
                 ctx.set_geometry(fluid)
                 ctx.init_fields(rho)
                 nf = ctx.info()["fluid_nodes"]

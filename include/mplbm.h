@@ -67,6 +67,13 @@ typedef enum mplbm_kernel {
  * fused multiply-adds and hardware reciprocal/exp. */
 typedef enum mplbm_math { MPLBM_MATH_REFERENCE = 0, MPLBM_MATH_FAST = 1 } mplbm_math;
 
+/* Halfway bounce-back in the FUSED kernel.  WRITE: after collision a node
+ * also stores f*_opp(i) into slot i of each solid upstream neighbour, so the
+ * next pull needs no mask test.  PULL: the pull reads the node's own
+ * opposite population where the link word marks a solid upstream node.
+ * Both give identical results. */
+typedef enum mplbm_bounce_back { MPLBM_BB_WRITE = 0, MPLBM_BB_PULL = 1 } mplbm_bounce_back;
+
 /* Outer-face types (index order xmin, xmax, ymin, ymax, zmin, zmax).  The
  * reference's face flags (include/global_var.cuh:39-63) are fixed to
  * periodic; PRESSURE prescribes face_rho[f] through the anti-bounce-back
@@ -100,6 +107,8 @@ typedef struct mplbm_config {
   int32_t tile_y, tile_z;      /* FUSED: threads per tile in y and z (0 = tuned default) */
   int32_t seg_x;               /* FUSED: planes marched per block (0 = tuned default) */
   int32_t plane_pad;           /* extra elements per population sub-plane (-1 = automatic) */
+  int32_t bounce_back;         /* mplbm_bounce_back: how FUSED resolves halfway bounce-back */
+  int32_t reserved2;
 } mplbm_config;
 
 /* FP64 reductions over fluid nodes (mplbm_reduce). */

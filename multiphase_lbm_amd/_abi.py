@@ -16,6 +16,7 @@ FP32, FP64, FP16S, BF16S = range(4)
 KERNEL_AUTO, KERNEL_FUSED, KERNEL_TWOPASS = range(3)
 FACE_PERIODIC, FACE_PRESSURE = 0, 1
 MATH_REFERENCE, MATH_FAST = 0, 1
+BB_WRITE, BB_PULL = 0, 1
 STEP_WRITE_MACRO = 1
 
 STORAGE_NAMES = {"fp32": FP32, "fp64": FP64, "fp16s": FP16S, "bf16s": BF16S}
@@ -36,6 +37,7 @@ class Config(ctypes.Structure):
         ("rho_ref", ctypes.c_double), ("dev_scale", ctypes.c_double),
         ("tile_y", ctypes.c_int32), ("tile_z", ctypes.c_int32),
         ("seg_x", ctypes.c_int32), ("plane_pad", ctypes.c_int32),
+        ("bounce_back", ctypes.c_int32), ("reserved2", ctypes.c_int32),
     ]
 
 

@@ -367,6 +367,7 @@ struct Tiling {
   int wrap_y, wrap_z;    // 1: the whole axis lies in one tile, neighbours wrap in LDS
   int nblocks;           // real blocks (grid may be padded to a multiple of 8)
   int xcd_remap;
+  int pull_bb;           // 1: bounce-back read on the pull side, no ghost stores
 };
 
 __device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
@@ -551,6 +552,12 @@ __device__ __forceinline__ void march_tile(const Lattice<Mo>& L, const Phys<type
         const Rsrc base = ex(i) == -1 ? bq : (ex(i) == 1 ? bm : b0);
         raw[i] = bload<S>(base, yzb[1 - ey(i)][1 - ez(i)], (uint32_t)i * psb);
       }
+      if (T.pull_bb && (lk & ~1u)) {
+        // halfway bounce-back read directly: B_i(x) = A_opp(i)(x)
+#pragma unroll
+        for (int i = 1; i < kQ; ++i)
+          if ((lk >> i) & 1u) raw[i] = bload<S>(b0, yzb[1][1], (uint32_t)opp(i) * psb);
+      }
     }
   };
   // Decode the pulled populations, apply face rules, return rho.
@@ -632,7 +639,7 @@ __device__ __forceinline__ void march_tile(const Lattice<Mo>& L, const Phys<type
       const Rsrc oq = make_rsrc(out_plane(xn));
 #pragma unroll
       for (int i = 0; i < kQ; ++i) bstore<S>(encode<Mo>(P[i], i, ph), o0, yzb[1][1], (uint32_t)i * psb);
-      if (lkP & ~1u) {
+      if (!T.pull_bb && (lkP & ~1u)) {
         // write-side bounce-back into the solid upstream nodes (see store_cell)
 #pragma unroll
         for (int i = 1; i < kQ; ++i)
@@ -1059,6 +1066,7 @@ static mplbm_status choose_tiling(mplbm_ctx* ctx) {
   if (nb > (1LL << 30)) return fail(ctx, MPLBM_ERR_ARG, "too many blocks");
   T.nblocks = (int)nb;
   T.xcd_remap = 1;
+  T.pull_bb = c.bounce_back == MPLBM_BB_PULL ? 1 : 0;
   ctx->tiling = T;
   return MPLBM_OK;
 }

@@ -42,7 +42,7 @@ def _run_pair(shape, fluid, rho, steps, storage="fp32", kernel="fused", step0=0,
     return st, (g_rho, g_p, g_u, g_f, stats)
 
 
-LIB_ONLY = ("tile_y", "tile_z", "seg_x", "rho_ref", "dev_scale")
+LIB_ONLY = ("tile_y", "tile_z", "seg_x", "rho_ref", "dev_scale", "bounce_back", "plane_pad")
 
 
 def _lib_params(params):
@@ -76,13 +76,13 @@ def test_fp32_small_bit_exact(kernel):
     assert np.array_equal(g_p[fluid], st.p[fluid].astype(np.float64))
 
 
-@pytest.mark.parametrize("kernel", ["fused", "twopass"])
-def test_fp32_config1_100_steps(kernel):
+@pytest.mark.parametrize("kernel,bb", [("fused", 0), ("fused", 1), ("twopass", 0)])
+def test_fp32_config1_100_steps(kernel, bb):
     """64^3, 40 spheres r in [3,6], G=-5, tau=1, rho0=0.7 +/- 0.01, 100 steps."""
     shape = (64, 64, 64)
     fluid, rho = _case(shape, 40, 3.0, 6.0, seed=1)
-    print(f"case fp32 config1 {kernel}", flush=True)
-    st, (g_rho, g_p, g_u, g_f, stats) = _run_pair(shape, fluid, rho, 100, "fp32", kernel, **SC)
+    print(f"case fp32 config1 {kernel} bb {bb}", flush=True)
+    st, (g_rho, g_p, g_u, g_f, stats) = _run_pair(shape, fluid, rho, 100, "fp32", kernel, bounce_back=bb, **SC)
     assert _relerr(g_rho[fluid], st.rho[fluid]) <= 1e-5
     umax = float(np.abs(st.u[fluid]).max())
     assert _abserr(g_u[fluid], st.u[fluid]) <= 1e-5 * umax
@@ -104,16 +104,17 @@ def test_fp64_config1_100_steps():
     assert abs(stats["mass"] - m0) <= 1e-12 * m0
 
 
+@pytest.mark.parametrize("bb", [0, 1])
 @pytest.mark.parametrize("tiles", [(6, 8, 3), (5, 16, 2), (32, 5, 7), (0, 0, 1)])
-def test_fused_tilings_agree(tiles):
-    """Ring tiles in both axes, partial tiles and short segments give the
-    oracle's result."""
+def test_fused_tilings_agree(tiles, bb):
+    """Ring tiles in both axes, partial tiles and short segments, with
+    write-side or pull-side bounce-back, give the oracle's result."""
     ty, tz, seg = tiles
     shape = (9, 23, 19)
     fluid, rho = _case(shape, 4, 1.5, 3.5, seed=7)
-    print(f"case tiling {tiles}", flush=True)
+    print(f"case tiling {tiles} bb {bb}", flush=True)
     st, (g_rho, g_p, g_u, g_f, _) = _run_pair(shape, fluid, rho, 12, "fp32", "fused", tile_y=ty, tile_z=tz,
-                                              seg_x=seg, **SC)
+                                              seg_x=seg, bounce_back=bb, **SC)
     assert np.array_equal(g_f[fluid], st.fA[fluid].astype(np.float64))
 
 
