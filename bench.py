@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--tile-z", type=int, default=0)
     ap.add_argument("--seg-x", type=int, default=0)
-    ap.add_argument("--bounce-back", type=int, default=0, help="0: write-side (ghost) stores, 1: pull-side reads")
+    ap.add_argument("--bounce-back", type=int, default=1, help="0: write-side (ghost) stores, 1: pull-side reads")
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline (0: skip)")
     ap.add_argument("--profile-only", action="store_true", help="run only the FP32 timed loop (for rocprofv3)")
     ap.add_argument("--precision", action="store_true", help="error study of FP32/FP16S/BF16S against FP64")
@@ -110,6 +110,7 @@ def run_gpu(a, shape, fluid, rho, storage):
         "bytes_per_update": 2 * Q * bpop,
         "fluid_nodes": nf,
         "tile": [info["tile_y"], info["tile_z"], info["seg_x"]],
+        "tag": f"{a.kernel}:{info['tile_y']}x{info['tile_z']}x{info['seg_x']}:bb{a.bounce_back}",
         "mass": stats["mass"],
         "nonfinite": stats["nonfinite"],
     }
@@ -132,16 +133,21 @@ def cpu_baseline(a, shape, fluid, rho):
                       f"of the same {shape[0]}^3 packing, {dt:.1f}s"}
 
 
-def traffic_from_profiles(storage):
+def traffic_from_profiles(storage, tag):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE +
-    WRITE_SIZE, scaled by the copy-kernel calibration) for this workload, as
-    committed in profiles/traffic_fused_512.json; None when absent."""
-    path = os.path.join(ROOT, "profiles", "traffic_fused_512.json")
+    WRITE_SIZE, each scaled by the copy-kernel calibration) for this
+    workload and kernel configuration, from the `traffic` lines of
+    profiles/traffic_fused_512.txt; None when not measured."""
+    path = os.path.join(ROOT, "profiles", "traffic_fused_512.txt")
     try:
         with open(path) as f:
-            return json.load(f).get(storage)
-    except (OSError, ValueError):
-        return None
+            for line in f:
+                w = line.split()
+                if len(w) == 4 and w[0] == "traffic" and w[1] == storage and w[2] == tag:
+                    return int(w[3])
+    except OSError:
+        pass
+    return None
 
 
 def precision_study(a, shape, fluid, rho):
@@ -300,7 +306,7 @@ def main():
                    "mlups_unit": "fluid-node updates per second / 1e6"},
         "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles("fp32"),
+                     "traffic": traffic_from_profiles("fp32", r["tag"]),
                      "bytes_per_update": r["bytes_per_update"], "kernel_ms": round(r["kernel_ms"], 4)},
         "cpu_baseline": cpu,
     }
@@ -311,7 +317,7 @@ def main():
         line[s] = {"value": round(q["mlups"], 1), "unit": "MLUPS", "vs_fp32": round(q["mlups"] / r["mlups"], 3),
                    "roofline": {"bound": "hbm", "achieved": round(q["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                                 "unit": "GB/s", "frac": round(q["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                                "traffic": traffic_from_profiles(s), "bytes_per_update": q["bytes_per_update"],
+                                "traffic": traffic_from_profiles(s, q["tag"]), "bytes_per_update": q["bytes_per_update"],
                                 "kernel_ms": round(q["kernel_ms"], 4)}}
     print(json.dumps(line), flush=True)
 

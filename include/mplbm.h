@@ -70,8 +70,8 @@ typedef enum mplbm_math { MPLBM_MATH_REFERENCE = 0, MPLBM_MATH_FAST = 1 } mplbm_
 /* Halfway bounce-back in the FUSED kernel.  WRITE: after collision a node
  * also stores f*_opp(i) into slot i of each solid upstream neighbour, so the
  * next pull needs no mask test.  PULL: the pull reads the node's own
- * opposite population where the link word marks a solid upstream node.
- * Both give identical results. */
+ * opposite population where the link word marks a solid upstream node
+ * (the default: fewer scattered stores).  Both give identical results. */
 typedef enum mplbm_bounce_back { MPLBM_BB_WRITE = 0, MPLBM_BB_PULL = 1 } mplbm_bounce_back;
 
 /* Outer-face types (index order xmin, xmax, ymin, ymax, zmin, zmax).  The

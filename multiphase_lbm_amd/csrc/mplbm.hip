@@ -1041,7 +1041,8 @@ static mplbm_status choose_tiling(mplbm_ctx* ctx) {
   if (tz <= 0) tz = c.lz <= 64 ? (int)c.lz : 34;
   if (tz >= c.lz) { T.wrap_z = 1; tz = (int)c.lz; }
   if (ty <= 0) {
-    ty = std::max(1, 384 / tz);
+    // about four waves per workgroup (measured best on 512^3 packings)
+    ty = std::max(1, 256 / tz);
     if (ty > 32) ty = 32;
   }
   if (ty >= c.ly && (int64_t)tz * c.ly <= 1024) { T.wrap_y = 1; ty = (int)c.ly; }
@@ -1056,7 +1057,7 @@ static mplbm_status choose_tiling(mplbm_ctx* ctx) {
   if (seg <= 0) {
     // long segments amortise the two warm-up planes; keep >= ~2 blocks per CU
     const int64_t tiles = (int64_t)T.tiles_y * T.tiles_z;
-    seg = 64;
+    seg = 32;
     while (seg > 4 && tiles * ((c.lx + seg - 1) / seg) < 1024) seg /= 2;
   }
   if (seg > c.lx) seg = (int)c.lx;
@@ -1143,6 +1144,7 @@ void mplbm_config_default(mplbm_config* c) {
   c->rhos = 0.0;
   c->inject_position = 1.0;
   c->plane_pad = -1;
+  c->bounce_back = MPLBM_BB_PULL;
 }
 
 const char* mplbm_last_error(const mplbm_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
