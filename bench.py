@@ -97,7 +97,7 @@ def run_gpu(a, shape, fluid, rho, storage):
     finally:
         ctx.close()
     nf = info["fluid_nodes"]
-    launches = a.steps * (1 if a.kernel == "fused" else 2)
+    launches = a.steps * (2 if a.kernel == "twopass" else 1)
     per_launch_s = ev_ms / 1e3 / launches
     bpop = mp.STORAGE_BYTES[mp.STORAGE_NAMES[storage.replace("fast", "")]]
     alg_bytes = 2 * Q * bpop * nf  # per step

@@ -51,14 +51,18 @@ typedef enum mplbm_storage {
   MPLBM_BF16S = 3
 } mplbm_storage;
 
-/* Step kernel.  FUSED: one marching stream-collide launch per step with the
- * pseudopotential halo in LDS.  TWOPASS: a density/psi launch followed by a
- * force/collision launch (the schedule the reference's do_collision needs
- * to be race-free). */
+/* Step kernel.  COMPACT (what AUTO selects): populations of fluid nodes
+ * only, one marching stream-collide launch per step that works through the
+ * fluid nodes of each plane, pseudopotential halo in LDS.  FUSED: the same
+ * schedule over the full lattice (every node stored, solid ones skipped).
+ * TWOPASS: full lattice, a density/psi launch followed by a force/collision
+ * launch (the schedule the reference's do_collision needs to be race-free).
+ * All three give identical results. */
 typedef enum mplbm_kernel {
   MPLBM_KERNEL_AUTO = 0,
   MPLBM_KERNEL_FUSED = 1,
-  MPLBM_KERNEL_TWOPASS = 2
+  MPLBM_KERNEL_TWOPASS = 2,
+  MPLBM_KERNEL_COMPACT = 3
 } mplbm_kernel;
 
 /* Arithmetic of the FP32 form.  REFERENCE evaluates the reference's
@@ -104,8 +108,9 @@ typedef struct mplbm_config {
                                   the Shan-Chen force (0 reproduces the reference) */
   double rho_ref;              /* FP16S/BF16S shift; 0 = choose from the initial field */
   double dev_scale;            /* FP16S/BF16S scale; 0 = choose automatically */
-  int32_t tile_y, tile_z;      /* FUSED: threads per tile in y and z (0 = tuned default) */
-  int32_t seg_x;               /* FUSED: planes marched per block (0 = tuned default) */
+  int32_t tile_y, tile_z;      /* FUSED: threads per tile in y and z incl. the ring; COMPACT:
+                                  interior nodes per tile in y and z (0 = tuned default) */
+  int32_t seg_x;               /* FUSED, COMPACT: planes marched per block (0 = tuned default) */
   int32_t plane_pad;           /* extra elements per population sub-plane (-1 = automatic) */
   int32_t bounce_back;         /* mplbm_bounce_back: how FUSED resolves halfway bounce-back */
   int32_t reserved2;
@@ -158,7 +163,8 @@ const char* mplbm_last_error(const mplbm_ctx* ctx);
 
 /* Solid mask, host uint8 [x][y][z], nonzero = fluid (the reference's
  * is_fluid, include/global_var.cuh:11).  Builds the per-node bounce-back
- * link words on the GPU. */
+ * link words on the GPU; for COMPACT also the plane ranks, the per-tile work
+ * lists and the population lattices sized to the fluid count. */
 mplbm_status mplbm_set_geometry(mplbm_ctx* ctx, const uint8_t* is_fluid);
 
 /* Replaces multiphaselbm_initialization + multiphaselbm_init
@@ -206,9 +212,10 @@ mplbm_status mplbm_write_populations_f64(mplbm_ctx* ctx, const double* f);
  * (synchronises). */
 mplbm_status mplbm_reduce(mplbm_ctx* ctx, mplbm_stats64* out);
 
-/* Changes the FUSED kernel's tile (threads in y and z, planes per block);
- * 0 keeps the tuned default for that field.  Used by the autotuner
- * (multiphase_lbm_amd/autotune.py); results do not depend on the tile. */
+/* Changes the FUSED or COMPACT kernel's tile (see mplbm_config.tile_y) and
+ * planes per block; 0 keeps the tuned default for that field.  Used by the
+ * autotuner (multiphase_lbm_amd/autotune.py); results do not depend on the
+ * tile. */
 mplbm_status mplbm_set_tiling(mplbm_ctx* ctx, int32_t tile_y, int32_t tile_z, int32_t seg_x);
 
 /* Context facts (no synchronisation). */

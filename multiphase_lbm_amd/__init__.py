@@ -12,13 +12,13 @@ import ctypes
 import numpy as np
 
 from . import _abi
-from ._abi import (BF16S, FACE_PERIODIC, FACE_PRESSURE, FP16S, FP32, FP64, KERNEL_AUTO, KERNEL_FUSED,
-                   KERNEL_TWOPASS, STEP_WRITE_MACRO, STORAGE_BYTES, STORAGE_NAMES)
+from ._abi import (BF16S, FACE_PERIODIC, FACE_PRESSURE, FP16S, FP32, FP64, KERNEL_AUTO, KERNEL_COMPACT,
+                   KERNEL_FUSED, KERNEL_TWOPASS, STEP_WRITE_MACRO, STORAGE_BYTES, STORAGE_NAMES)
 
 __all__ = ["Context", "MplbmError", "FP32", "FP64", "FP16S", "BF16S", "KERNEL_AUTO", "KERNEL_FUSED",
-           "KERNEL_TWOPASS", "FACE_PERIODIC", "FACE_PRESSURE", "default_config"]
+           "KERNEL_TWOPASS", "KERNEL_COMPACT", "FACE_PERIODIC", "FACE_PRESSURE", "default_config"]
 
-KERNEL_NAMES = {"auto": KERNEL_AUTO, "fused": KERNEL_FUSED, "twopass": KERNEL_TWOPASS}
+KERNEL_NAMES = {"auto": KERNEL_AUTO, "fused": KERNEL_FUSED, "twopass": KERNEL_TWOPASS, "compact": KERNEL_COMPACT}
 
 
 class MplbmError(RuntimeError):

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(PKG, "libmplbm.so")
 
 OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_NOMEM, ERR_UNSUPPORTED = range(6)
 FP32, FP64, FP16S, BF16S = range(4)
-KERNEL_AUTO, KERNEL_FUSED, KERNEL_TWOPASS = range(3)
+KERNEL_AUTO, KERNEL_FUSED, KERNEL_TWOPASS, KERNEL_COMPACT = range(4)
 FACE_PERIODIC, FACE_PRESSURE = 0, 1
 MATH_REFERENCE, MATH_FAST = 0, 1
 BB_WRITE, BB_PULL = 0, 1

@@ -358,7 +358,29 @@ struct Lattice {
   uint32_t plane;        // ly*lz (cells of an x-plane; field arrays use it)
   uint32_t ps;           // population sub-plane stride in elements (>= plane)
   uint32_t ncell;
+  // compact layout (fluid nodes only): see cpix
+  const uint2* rw;       // [x][y][w]: {fluid bits of z = 32w..32w+31, plane rank of z = 32w}
+  uint32_t W;            // words per (x, y) row
+  int compact;
 };
+
+// Compact layout.  Only fluid nodes are stored.  The fluid nodes of an
+// x-plane are numbered in (y, z) raster order (their plane rank r); population
+// i of the node of rank r in plane x lives at element ((x+1)*19 + i)*ps + r,
+// ps >= the largest plane count.  Memory planes 0 and lx+1 are copies of
+// planes lx-1 and 0 (kept current by the step kernel), so the three source
+// planes of a pull are always adjacent in memory.
+__device__ __forceinline__ uint32_t plane_rank(const uint2* rw, uint32_t W, int ly, int x, int y, int z) {
+  const uint2 w = rw[((uint64_t)x * ly + y) * W + ((uint32_t)z >> 5)];
+  return w.y + __popc(w.x & ((1u << (z & 31)) - 1u));
+}
+__device__ __forceinline__ bool rw_fluid(const uint2* rw, uint32_t W, int ly, int x, int y, int z) {
+  return (rw[((uint64_t)x * ly + y) * W + ((uint32_t)z >> 5)].x >> (z & 31)) & 1u;
+}
+template <class Mo>
+__device__ __forceinline__ uint64_t cpix(const Lattice<Mo>& L, int i, int mplane, uint32_t rank) {
+  return ((uint64_t)mplane * kQ + i) * L.ps + rank;
+}
 
 struct Tiling {
   int TY, TZ;            // threads per tile in y and z (ring included unless wrapped)
@@ -788,6 +810,261 @@ __global__ void k_ghost(Lattice<Mo> L) {
 }
 
 // ----------------------------------------------------------------------------
+// compact marching kernel (fluid nodes only)
+
+struct CTiling {
+  int TYi, TZi;          // interior nodes of a tile in y and z
+  int PZ, RZ;            // row lengths of the psi region (TZi+2) and the rank region (TZi+4)
+  int PYZ, RYZ;          // sizes of the psi region (TYi+2)(TZi+2) and the rank region (TYi+4)(TZi+4)
+  uint32_t magic_pz, magic_rz;  // ceil(2^32/PZ), ceil(2^32/RZ): q/PZ = umulhi(q, magic) for q < 2^16
+  int tiles_y, tiles_z;
+  int seg, nseg;         // planes per workgroup, segments along x
+  int nblocks;
+  int xcd_remap;
+  int threads;           // workgroup size (>= TYi*TZi)
+};
+
+struct CLists {
+  const uint16_t* pos;   // psi-region positions of the nodes a tile pulls in a plane, interior first
+  const uint2* tab;      // [tile][x]: {first entry, n_interior | n_pull << 16}
+};
+
+// One workgroup marches a (y, z) tile of TYi x TZi nodes along the planes of
+// an x segment.  Three regions of each plane are staged in LDS:
+//  * rank region (tile + 2-node ring): byte offset of each fluid node within
+//    its population sub-plane (plane rank * element size), ~0 for solid;
+//  * psi region (tile + 1-node ring): psi of every node, solid ones included.
+// Per plane x the workgroup (1) builds the rank region of plane x+3 and the
+// solid part of its psi region, (2) pulls every fluid node of the psi region
+// of plane x+1 (work list: the fluid nodes of that plane, not the thread
+// grid) and writes its psi, keeping the populations of interior nodes in
+// registers, (3) barrier, (4) evaluates the force of the fluid interior
+// nodes of plane x from the psi ring, collides them and stores.  Threads take
+// list entries tid, tid + kThreads, ..., so the work follows the fluid count
+// of each plane.  Eight LDS slots per region (plane x' in slot
+// (x' - x0 + 2) & 7) keep every read at least one barrier away from the next
+// write of its slot.
+template <class Mo, bool kFaces, int kThreads>
+__global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typename Mo::C> ph, CTiling T,
+                                                      CLists W) {
+  using C = typename Mo::C;
+  using S = typename Mo::S;
+  extern __shared__ unsigned char smem_raw[];
+  C* psl = reinterpret_cast<C*>(smem_raw);                                   // 8 x PYZ
+  uint32_t* rm = reinterpret_cast<uint32_t*>(smem_raw + 8 * T.PYZ * sizeof(C));  // 8 x RYZ
+  int* ycrd = reinterpret_cast<int*>(rm + 8 * T.RYZ);                        // TYi + 4 rows
+  int* zcrd = ycrd + (T.TYi + 4);                                            // TZi + 4 columns
+
+  int b = blockIdx.x;
+  if (T.xcd_remap) {
+    const int per = gridDim.x >> 3;
+    b = (b & 7) * per + (b >> 3);
+  }
+  if (b >= T.nblocks) return;  // whole padding block leaves before any barrier
+  const int ntile = T.tiles_y * T.tiles_z;
+  const int sg = b / ntile;
+  const int tile = b - sg * ntile;
+  const int tyi = tile / T.tiles_z;
+  const int tzi = tile - tyi * T.tiles_z;
+  const int y0 = tyi * T.TYi, z0 = tzi * T.TZi;
+  const int x0 = __builtin_amdgcn_readfirstlane(sg * T.seg);
+  const int x1 = __builtin_amdgcn_readfirstlane(min(x0 + T.seg, L.lx));
+  const int n = x1 - x0;
+  const int tid = threadIdx.x;
+
+  constexpr uint32_t sz = sizeof(S);
+  const uint32_t psb = L.ps * sz;  // bytes of a population sub-plane
+  const uint32_t plb = kQ * psb;   // bytes of a plane
+  const C psi_solid = psi_of<Mo>(ph, ph.rhos);
+
+  // global coordinates of the rank region's rows and columns
+  for (int q = tid; q < T.TYi + 4; q += kThreads) ycrd[q] = ((y0 - 2 + q) % L.ly + L.ly) % L.ly;
+  for (int q = tid; q < T.TZi + 4; q += kThreads) zcrd[q] = ((z0 - 2 + q) % L.lz + L.lz) % L.lz;
+  __syncthreads();
+
+  // planes x0-2 .. x1+1 are addressed; with lx < 3 one wrap is not enough
+  auto wrapx = [&](int v) {
+    const int r = v % L.lx;
+    return r < 0 ? r + L.lx : r;
+  };
+  auto slot = [&](int xp) { return (xp - x0 + 2) & 7; };
+
+  // (1) rank region and solid psi of plane xp
+  auto expand = [&](int xp) {
+    const uint2* rwp = L.rw + (uint64_t)wrapx(xp) * L.ly * L.W;
+    uint32_t* m = rm + slot(xp) * T.RYZ;
+    C* pz = psl + slot(xp) * T.PYZ;
+    for (int q = tid; q < T.RYZ; q += kThreads) {
+      const int r = (int)__umulhi((uint32_t)q, T.magic_rz);
+      const int c = q - r * T.RZ;
+      const int z = zcrd[c];
+      const uint2 w = rwp[(uint32_t)ycrd[r] * L.W + ((uint32_t)z >> 5)];
+      const uint32_t bit = (uint32_t)z & 31u;
+      const bool fl = (w.x >> bit) & 1u;
+      m[q] = fl ? (w.y + __popc(w.x & ((1u << bit) - 1u))) * sz : ~0u;
+      if (!fl && r >= 1 && r <= T.TYi + 2 && c >= 1 && c <= T.TZi + 2) pz[(r - 1) * T.PZ + (c - 1)] = psi_solid;
+    }
+  };
+
+  // Pull of the fluid node at psi position p of plane xp: populations f,
+  // face rules, density.  Sources in planes xp-1, xp, xp+1 lie in memory
+  // planes xw, xw+1, xw+2 (xw = xp wrapped), one buffer resource.  A solid
+  // source gives the halfway bounce-back B_i = A_opp(i) of the node itself
+  // (iteration_kernel.cuh:112-115).
+  auto pull = [&](int xp, int p, C* f, int& gy, int& gz, uint32_t& self) -> C {
+    const int xw = wrapx(xp);
+    const int r = (int)__umulhi((uint32_t)p, T.magic_pz);
+    const int c = p - r * T.PZ;
+    const int q0 = (r + 1) * T.RZ + (c + 1);
+    gy = ycrd[r + 1];
+    gz = zcrd[c + 1];
+    const uint32_t* m0 = rm + slot(xp) * T.RYZ + q0;
+    const uint32_t* mm = rm + slot(xp - 1) * T.RYZ + q0;
+    const uint32_t* mq = rm + slot(xp + 1) * T.RYZ + q0;
+    self = m0[0];
+    const uint32_t own = self + plb;  // own node, relative to memory plane xw
+    const Rsrc R = make_rsrc(L.fin + (uint64_t)xw * kQ * L.ps);
+    S raw[kQ];
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) {
+      const uint32_t* mi = ex(i) == 1 ? mm : (ex(i) == -1 ? mq : m0);
+      const uint32_t sr = mi[-ey(i) * T.RZ - ez(i)];
+      const uint32_t voff = sr != ~0u ? sr + (uint32_t)(1 - ex(i)) * plb + (uint32_t)i * psb
+                                      : own + (uint32_t)opp(i) * psb;
+      raw[i] = bload<S>(R, voff, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) f[i] = decode<Mo>(raw[i], i, ph);
+    if constexpr (kFaces) {
+      if (on_face(ph, xw, gy, gz, L.lx, L.ly, L.lz)) {
+        C a[kQ];
+#pragma unroll
+        for (int k = 0; k < kQ; ++k) a[k] = decode<Mo>(bload<S>(R, own + (uint32_t)k * psb, 0), k, ph);
+#pragma unroll
+        for (int i = 1; i < kQ; ++i) {
+          const int fc = entering_face(ph, i, xw, gy, gz, L.lx, L.ly, L.lz);
+          if (fc >= 0) f[i] = abb_value<Mo>(a, i, ph.face_rho[fc]);
+        }
+      }
+    }
+    return density(f, gy, ph);
+  };
+
+  // (2) psi of every fluid node of plane xp's psi region.  Interior nodes
+  // come first in a list and number at most kThreads, so entry tid is this
+  // thread's interior node when tid < n_interior: its populations, density
+  // and position stay in registers (the *N set) until the collision of
+  // plane xp one iteration later.  Entries from kThreads on are ring nodes
+  // and are pulled first, while only the previous plane's set is live.
+  C fN[kQ], rhoN = (C)0;
+  int pN = 0, gyN = 0, gzN = 0;
+  uint32_t ownN = 0;
+  bool hasN = false;
+  auto psi_plane = [&](int xp, bool keep) {
+    const uint2 tb = W.tab[(uint64_t)tile * L.lx + wrapx(xp)];
+    const uint32_t first = __builtin_amdgcn_readfirstlane(tb.x);
+    const int nint = (int)__builtin_amdgcn_readfirstlane(tb.y & 0xFFFFu);
+    const int npull = (int)__builtin_amdgcn_readfirstlane(tb.y >> 16);
+    C* pz = psl + slot(xp) * T.PYZ;
+    for (int k = tid + kThreads; k < npull; k += kThreads) {
+      const int p = W.pos[first + k];
+      C f[kQ];
+      int gy, gz;
+      uint32_t self;
+      const C rho = pull(xp, p, f, gy, gz, self);
+      pz[p] = psi_of<Mo>(ph, rho);
+    }
+    hasN = false;
+    if (tid < npull) {
+      pN = W.pos[first + tid];
+      rhoN = pull(xp, pN, fN, gyN, gzN, ownN);
+      pz[pN] = psi_of<Mo>(ph, rhoN);
+      hasN = keep && tid < nint;
+    }
+  };
+
+  // prologue: rank regions of planes x0-2 .. x0+2, psi of planes x0-1, x0
+  for (int k = -2; k <= 2; ++k) expand(x0 + k);
+  __syncthreads();
+  psi_plane(x0 - 1, false);
+  psi_plane(x0, true);
+
+  for (int j = 0; j < n; ++j) {
+    const int x = __builtin_amdgcn_readfirstlane(x0 + j);
+    // this thread's interior node of plane x
+    C f[kQ];
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) f[i] = fN[i];
+    const C rho = rhoN;
+    const int p = pN, gy = gyN, gz = gzN;
+    const uint32_t own = ownN;
+    const bool has = hasN;
+
+    if (j + 2 <= n) expand(x + 3);  // needed up to plane x1 + 1
+    psi_plane(x + 1, true);
+    __syncthreads();
+
+    // (4) collide this thread's fluid interior node of plane x
+    if (has) {
+      const int xw = wrapx(x);
+      const C* pzm = psl + slot(x - 1) * T.PYZ;
+      const C* pz0 = psl + slot(x) * T.PYZ;
+      const C* pzq = psl + slot(x + 1) * T.PYZ;
+      C F[3];
+      bool skip = false;
+      if constexpr (kFaces) skip = force_skipped(ph, xw, gy, gz, L.lx, L.ly, L.lz);
+      if (skip) {
+        F[0] = F[1] = F[2] = (C)0;
+      } else {
+        C pn[kQ];
+        pn[0] = (C)0;
+#pragma unroll
+        for (int i = 1; i < kQ; ++i) {
+          const C* pp = ex(i) == 1 ? pzq : (ex(i) == -1 ? pzm : pz0);
+          pn[i] = pp[p + ey(i) * T.PZ + ez(i)];
+        }
+        sc_force(pn, pz0[p], ph, F);
+      }
+      C u[3];
+      collide<Mo>(f, rho, F, ph, u);
+      const Rsrc O = make_rsrc(L.fout + (uint64_t)(xw + 1) * kQ * L.ps);
+#pragma unroll
+      for (int i = 0; i < kQ; ++i) bstore<S>(encode<Mo>(f[i], i, ph), O, own, (uint32_t)i * psb);
+      if (xw == 0 || xw == L.lx - 1) {  // ghost copies of the first and last planes
+        if (xw == 0) {
+          const Rsrc G = make_rsrc(L.fout + (uint64_t)(L.lx + 1) * kQ * L.ps);
+#pragma unroll
+          for (int i = 0; i < kQ; ++i) bstore<S>(encode<Mo>(f[i], i, ph), G, own, (uint32_t)i * psb);
+        }
+        if (xw == L.lx - 1) {
+          const Rsrc G = make_rsrc(L.fout);
+#pragma unroll
+          for (int i = 0; i < kQ; ++i) bstore<S>(encode<Mo>(f[i], i, ph), G, own, (uint32_t)i * psb);
+        }
+      }
+      if (ph.write_macro) {
+        const uint64_t self = ((uint64_t)xw * L.ly + gy) * L.lz + gz;
+        L.rho[self] = (typename Mo::M)rho;
+        L.pres[self] = (typename Mo::M)pressure_of<Mo>(ph, rho);
+        L.u[self] = (typename Mo::M)u[0];
+        L.u[(uint64_t)L.ncell + self] = (typename Mo::M)u[1];
+        L.u[2ull * L.ncell + self] = (typename Mo::M)u[2];
+      }
+    }
+  }
+}
+
+// Store population i of the fluid node of rank r in plane x of a compact
+// lattice, and its copy in the ghost plane when x is the first or last plane.
+template <class Mo>
+__device__ __forceinline__ void compact_store_all(const Lattice<Mo>& L, typename Mo::S* lat, int i, int x,
+                                                  uint32_t r, typename Mo::S v) {
+  lat[cpix(L, i, x + 1, r)] = v;
+  if (x == 0) lat[cpix(L, i, L.lx + 1, r)] = v;
+  if (x == L.lx - 1) lat[cpix(L, i, 0, r)] = v;
+}
+
+// ----------------------------------------------------------------------------
 // set-up kernels
 
 // link word: bit 0 solid cell, bit i upstream node x - e_i solid
@@ -818,11 +1095,23 @@ __global__ void k_init(Lattice<Mo> L, typename Mo::S* fother, Phys<typename Mo::
   if (c >= L.ncell) return;
   const C rho = (C)rho_in[c];
   const bool fluid = !(L.link[c] & 1u);
+  if (!L.compact) {
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) {
-    const typename Mo::S v = encode<Mo>(weight<C>(i) * rho, i, ph);
-    L.fout[L.pix(i, c)] = v;
-    fother[L.pix(i, c)] = v;
+    for (int i = 0; i < kQ; ++i) {
+      const typename Mo::S v = encode<Mo>(weight<C>(i) * rho, i, ph);
+      L.fout[L.pix(i, c)] = v;
+      fother[L.pix(i, c)] = v;
+    }
+  } else if (fluid) {
+    int x, y, z;
+    cell_coords(L, c, x, y, z);
+    const uint32_t r = plane_rank(L.rw, L.W, L.ly, x, y, z);
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) {
+      const typename Mo::S v = encode<Mo>(weight<C>(i) * rho, i, ph);
+      compact_store_all(L, L.fout, i, x, r, v);
+      compact_store_all(L, fother, i, x, r, v);
+    }
   }
   if (fluid) {
     L.rho[c] = (typename Mo::M)rho;
@@ -839,12 +1128,27 @@ __global__ void k_init(Lattice<Mo> L, typename Mo::S* fother, Phys<typename Mo::
 }
 
 // decode the current lattice into [cell][19] doubles / encode back
+// element index of population i of cell c in either layout (compact: fluid
+// cells only)
+template <class Mo>
+__device__ __forceinline__ uint64_t any_pix(const Lattice<Mo>& L, int i, uint32_t c) {
+  if (!L.compact) return L.pix(i, c);
+  int x, y, z;
+  cell_coords(L, c, x, y, z);
+  return cpix(L, i, x + 1, plane_rank(L.rw, L.W, L.ly, x, y, z));
+}
+
 template <class Mo>
 __global__ void k_read_pops(Lattice<Mo> L, Phys<typename Mo::C> ph, double* out) {
   const uint32_t c = blockIdx.x * 256u + threadIdx.x;
   if (c >= L.ncell) return;
+  if (L.compact && (L.link[c] & 1u)) {  // no storage for solid nodes
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) out[(uint64_t)c * kQ + i] = (double)decode<Mo>(L.fin[L.pix(i, c)], i, ph);
+    for (int i = 0; i < kQ; ++i) out[(uint64_t)c * kQ + i] = 0.0;
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) out[(uint64_t)c * kQ + i] = (double)decode<Mo>(L.fin[any_pix(L, i, c)], i, ph);
 }
 
 template <class Mo>
@@ -852,8 +1156,17 @@ __global__ void k_write_pops(Lattice<Mo> L, Phys<typename Mo::C> ph, const doubl
   using C = typename Mo::C;
   const uint32_t c = blockIdx.x * 256u + threadIdx.x;
   if (c >= L.ncell) return;
+  if (!L.compact) {
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) L.fout[L.pix(i, c)] = encode<Mo>((C)in[(uint64_t)c * kQ + i], i, ph);
+    for (int i = 0; i < kQ; ++i) L.fout[L.pix(i, c)] = encode<Mo>((C)in[(uint64_t)c * kQ + i], i, ph);
+    return;
+  }
+  if (L.link[c] & 1u) return;
+  int x, y, z;
+  cell_coords(L, c, x, y, z);
+  const uint32_t r = plane_rank(L.rw, L.W, L.ly, x, y, z);
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) compact_store_all(L, L.fout, i, x, r, encode<Mo>((C)in[(uint64_t)c * kQ + i], i, ph));
 }
 
 // ----------------------------------------------------------------------------
@@ -875,7 +1188,7 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(Lattice<Mo> L, Phys<type
     double s = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
 #pragma unroll
     for (int i = 0; i < kQ; ++i) {
-      const double f = (double)decode<Mo>(L.fin[L.pix(i, c)], i, ph);
+      const double f = (double)decode<Mo>(L.fin[any_pix(L, i, c)], i, ph);
       if (!(f - f == 0.0)) acc[15] += 1.0;
       s += f;
       mx += f * ex(i);
@@ -948,6 +1261,16 @@ struct mplbm_ctx {
   int kernel = MPLBM_KERNEL_FUSED;
   float rho_ref = 0.f, scale = 1.f;
   std::string err;
+  // compact layout
+  bool compact = false;
+  uint2* rw = nullptr;          // plane-rank words
+  uint32_t W = 0;               // words per row
+  uint16_t* cpos = nullptr;     // work-list entries
+  uint2* ctab = nullptr;        // [tile][x] list table
+  size_t cpos_cap = 0, ctab_cap = 0;
+  CTiling ctiling{};
+  std::vector<uint8_t> hmask;   // host copy of the solid mask (list rebuilds)
+  size_t lat_bytes = 0;
 };
 
 static std::string g_create_error;
@@ -1029,6 +1352,9 @@ static Lattice<Mo> make_lattice(const mplbm_ctx* ctx, int in, int out) {
   L.plane = (uint32_t)(ctx->cfg.ly * ctx->cfg.lz);
   L.ps = (uint32_t)ctx->ps;
   L.ncell = (uint32_t)ctx->ncell;
+  L.rw = ctx->rw;
+  L.W = ctx->W;
+  L.compact = ctx->compact ? 1 : 0;
   return L;
 }
 
@@ -1074,12 +1400,188 @@ static mplbm_status choose_tiling(mplbm_ctx* ctx) {
 
 static int fused_threads(const Tiling& T) { return ((T.TY * T.TZ + 63) / 64) * 64; }
 
+// Tile of the compact kernel: TZi x TYi interior nodes (z a multiple of 32
+// where the axis allows, so a tile row covers whole rank words), segments of
+// seg planes; about 1024 workgroups or more.
+constexpr int kCompactMaxThreads = 512;  // workgroup size >= interior nodes of a tile
+
+static mplbm_status choose_ctiling(mplbm_ctx* ctx) {
+  const mplbm_config& c = ctx->cfg;
+  CTiling T{};
+  int tz = c.tile_z, ty = c.tile_y, seg = c.seg_x;
+  if (tz <= 0) tz = c.lz <= 32 ? (int)c.lz : 32;
+  if (ty <= 0) ty = c.ly <= 8 ? (int)c.ly : 8;
+  if (tz > c.lz) tz = (int)c.lz;
+  if (ty > c.ly) ty = (int)c.ly;
+  if (tz < 1 || ty < 1) return fail(ctx, MPLBM_ERR_ARG, "tile %dx%d too small", ty, tz);
+  if ((int64_t)ty * tz > kCompactMaxThreads)
+    return fail(ctx, MPLBM_ERR_ARG, "tile %dx%d exceeds %d interior nodes", ty, tz, kCompactMaxThreads);
+  T.TYi = ty;
+  T.TZi = tz;
+  T.PZ = tz + 2;
+  T.RZ = tz + 4;
+  T.PYZ = (ty + 2) * (tz + 2);
+  T.RYZ = (ty + 4) * (tz + 4);
+  T.magic_pz = (uint32_t)(((1ull << 32) + T.PZ - 1) / T.PZ);
+  T.magic_rz = (uint32_t)(((1ull << 32) + T.RZ - 1) / T.RZ);
+  T.tiles_y = (int)((c.ly + ty - 1) / ty);
+  T.tiles_z = (int)((c.lz + tz - 1) / tz);
+  {
+    const size_t csz = c.storage == MPLBM_FP64 ? 8 : 4;
+    const size_t lds = 8 * (size_t)T.PYZ * csz + 8 * (size_t)T.RYZ * 4 + (size_t)(ty + 4 + tz + 4) * 4;
+    if (lds > 65536) return fail(ctx, MPLBM_ERR_ARG, "tile %dx%d needs %zu bytes of LDS (max 65536)", ty, tz, lds);
+  }
+  const int64_t tiles = (int64_t)T.tiles_y * T.tiles_z;
+  if (seg <= 0) {
+    seg = 64;
+    while (seg > 4 && tiles * ((c.lx + seg - 1) / seg) < 2048) seg /= 2;
+  }
+  if (seg > c.lx) seg = (int)c.lx;
+  T.seg = seg;
+  T.nseg = (int)((c.lx + seg - 1) / seg);
+  const int64_t nb = tiles * T.nseg;
+  if (nb > (1LL << 30)) return fail(ctx, MPLBM_ERR_ARG, "too many blocks");
+  T.nblocks = (int)nb;
+  T.xcd_remap = 1;
+  T.threads = ty * tz <= 256 ? 256 : kCompactMaxThreads;
+  ctx->ctiling = T;
+  return MPLBM_OK;
+}
+
+template <class C>
+static size_t compact_lds_bytes(const CTiling& T) {
+  return 8 * (size_t)T.PYZ * sizeof(C) + 8 * (size_t)T.RYZ * 4 + (size_t)(T.TYi + 4 + T.TZi + 4) * 4;
+}
+
+// Work lists of the compact kernel: for every tile and plane, the psi-region
+// positions (r, c) of the fluid nodes the tile pulls, interior nodes first
+// (the ones it collides), then the ring.  The region covers the tile's
+// in-domain interior [1, ny] x [1, nz] and the ring of one node around it.
+static mplbm_status build_lists(mplbm_ctx* ctx) {
+  const CTiling& T = ctx->ctiling;
+  const int64_t lx = ctx->cfg.lx, ly = ctx->cfg.ly, lz = ctx->cfg.lz;
+  const int64_t ntile = (int64_t)T.tiles_y * T.tiles_z;
+  std::vector<uint2> tab((size_t)(ntile * lx));
+  std::vector<uint16_t> pos;
+  pos.reserve((size_t)(ctx->fluid_nodes * 3 / 2 + 1024));
+  const uint8_t* m = ctx->hmask.data();
+  std::vector<uint16_t> ring;
+  for (int64_t t = 0; t < ntile; ++t) {
+    const int64_t tyi = t / T.tiles_z, tzi = t - tyi * T.tiles_z;
+    const int64_t y0 = tyi * T.TYi, z0 = tzi * T.TZi;
+    const int ny = (int)std::min<int64_t>(T.TYi, ly - y0), nz = (int)std::min<int64_t>(T.TZi, lz - z0);
+    for (int64_t x = 0; x < lx; ++x) {
+      const size_t first = pos.size();
+      ring.clear();
+      for (int r = 0; r <= ny + 1; ++r) {
+        const int64_t y = ((y0 - 1 + r) % ly + ly) % ly;
+        const uint8_t* row = m + ((size_t)x * ly + y) * lz;
+        for (int c = 0; c <= nz + 1; ++c) {
+          const int64_t z = ((z0 - 1 + c) % lz + lz) % lz;
+          if (!row[z]) continue;
+          const uint16_t p = (uint16_t)(r * T.PZ + c);
+          if (r >= 1 && r <= ny && c >= 1 && c <= nz) pos.push_back(p);
+          else ring.push_back(p);
+        }
+      }
+      const size_t nint = pos.size() - first;
+      pos.insert(pos.end(), ring.begin(), ring.end());
+      const size_t npull = pos.size() - first;
+      if (first > 0xFFFFFFFFull) return fail(ctx, MPLBM_ERR_ARG, "work lists exceed 2^32 entries");
+      tab[(size_t)(t * lx + x)] = make_uint2((uint32_t)first, (uint32_t)nint | ((uint32_t)npull << 16));
+    }
+  }
+  pos.push_back(0);  // keeps the allocation non-empty
+  if (pos.size() > ctx->cpos_cap) {
+    if (ctx->cpos) HIP_TRY(ctx, hipFree(ctx->cpos));
+    ctx->cpos = nullptr;
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->cpos, pos.size() * 2));
+    ctx->cpos_cap = pos.size();
+  }
+  if (tab.size() > ctx->ctab_cap) {
+    if (ctx->ctab) HIP_TRY(ctx, hipFree(ctx->ctab));
+    ctx->ctab = nullptr;
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->ctab, tab.size() * sizeof(uint2)));
+    ctx->ctab_cap = tab.size();
+  }
+  HIP_TRY(ctx, hipMemcpy(ctx->cpos, pos.data(), pos.size() * 2, hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(ctx->ctab, tab.data(), tab.size() * sizeof(uint2), hipMemcpyHostToDevice));
+  return MPLBM_OK;
+}
+
+// Plane ranks, sub-plane stride and lattices of the compact layout.
+static mplbm_status build_compact(mplbm_ctx* ctx) {
+  const int64_t lx = ctx->cfg.lx, ly = ctx->cfg.ly, lz = ctx->cfg.lz;
+  const uint32_t W = (uint32_t)((lz + 31) / 32);
+  std::vector<uint2> rw((size_t)(lx * ly * W));
+  uint64_t maxc = 0;
+  const uint8_t* m = ctx->hmask.data();
+  // plane rank order: 32-wide z blocks outermost, then y, then z, so the rows
+  // of a tile 32 nodes wide form one contiguous range of a plane
+  for (int64_t x = 0; x < lx; ++x) {
+    uint32_t r = 0;
+    for (uint32_t w = 0; w < W; ++w) {
+      for (int64_t y = 0; y < ly; ++y) {
+        const uint8_t* row = m + ((size_t)x * ly + y) * lz;
+        uint32_t bits = 0;
+        for (int b = 0; b < 32; ++b) {
+          const int64_t z = (int64_t)w * 32 + b;
+          if (z < lz && row[z]) bits |= 1u << b;
+        }
+        rw[(size_t)((x * ly + y) * W + w)] = make_uint2(bits, r);
+        r += (uint32_t)__builtin_popcount(bits);
+      }
+    }
+    maxc = std::max<uint64_t>(maxc, r);
+  }
+  uint64_t ps = std::max<uint64_t>(64, (maxc + 63) / 64 * 64);
+  int64_t pad = ctx->cfg.plane_pad;
+  if (pad < 0) pad = (ps % 4096 == 0) ? 64 : 0;
+  ps += (uint64_t)pad;
+  // a pull addresses three planes from one 32-bit byte offset
+  if (3ull * kQ * ps * (uint64_t)ctx->bytes_pop >= (1ull << 32))
+    return fail(ctx, MPLBM_ERR_ARG, "planes too large for the compact layout");
+  const size_t lat_bytes = (size_t)(lx + 2) * kQ * ps * ctx->bytes_pop;
+  for (int k = 0; k < 2; ++k) {
+    if (ctx->lat[k]) HIP_TRY(ctx, hipFree(ctx->lat[k]));
+    ctx->lat[k] = nullptr;
+  }
+  if (ctx->rw) HIP_TRY(ctx, hipFree(ctx->rw));
+  ctx->rw = nullptr;
+  ctx->device_bytes -= (int64_t)(2 * ctx->lat_bytes);
+  HIP_TRY(ctx, hipMalloc(&ctx->lat[0], lat_bytes));
+  HIP_TRY(ctx, hipMalloc(&ctx->lat[1], lat_bytes));
+  ctx->lat_bytes = lat_bytes;
+  ctx->device_bytes += (int64_t)(2 * lat_bytes);
+  HIP_TRY(ctx, hipMalloc((void**)&ctx->rw, rw.size() * sizeof(uint2)));
+  HIP_TRY(ctx, hipMemcpy(ctx->rw, rw.data(), rw.size() * sizeof(uint2), hipMemcpyHostToDevice));
+  ctx->W = W;
+  ctx->ps = ps;
+  return build_lists(ctx);
+}
+
 template <class Mo>
 static mplbm_status launch_step(mplbm_ctx* ctx, int64_t step, int32_t reverse, uint32_t inject_type, bool macro) {
   using C = typename Mo::C;
   const Phys<C> ph = make_phys<C>(ctx, step, reverse, inject_type, macro);
   const Lattice<Mo> L = make_lattice<Mo>(ctx, ctx->cur, 1 - ctx->cur);
-  if (ctx->kernel == MPLBM_KERNEL_FUSED) {
+  if (ctx->kernel == MPLBM_KERNEL_COMPACT) {
+    const CTiling& T = ctx->ctiling;
+    const int grid = T.xcd_remap ? ((T.nblocks + 7) / 8) * 8 : T.nblocks;
+    const size_t shm = compact_lds_bytes<C>(T);
+    const CLists WL{ctx->cpos, ctx->ctab};
+    if (T.threads == 256) {
+      if (ph.any_face)
+        hipLaunchKernelGGL((k_compact<Mo, true, 256>), dim3(grid), dim3(256), shm, ctx->stream, L, ph, T, WL);
+      else
+        hipLaunchKernelGGL((k_compact<Mo, false, 256>), dim3(grid), dim3(256), shm, ctx->stream, L, ph, T, WL);
+    } else {
+      if (ph.any_face)
+        hipLaunchKernelGGL((k_compact<Mo, true, 512>), dim3(grid), dim3(512), shm, ctx->stream, L, ph, T, WL);
+      else
+        hipLaunchKernelGGL((k_compact<Mo, false, 512>), dim3(grid), dim3(512), shm, ctx->stream, L, ph, T, WL);
+    }
+  } else if (ctx->kernel == MPLBM_KERNEL_FUSED) {
     const Tiling& T = ctx->tiling;
     const int threads = fused_threads(T);
     const int grid = T.xcd_remap ? ((T.nblocks + 7) / 8) * 8 : T.nblocks;
@@ -1168,12 +1670,15 @@ mplbm_status mplbm_create(const mplbm_config* cfg, mplbm_ctx** out) {
   }
   for (int f = 0; f < 6; ++f)
     if (cfg->face_type[f] < 0 || cfg->face_type[f] > 1) return fail(nullptr, MPLBM_ERR_ARG, "bad face type");
+  if (cfg->kernel < MPLBM_KERNEL_AUTO || cfg->kernel > MPLBM_KERNEL_COMPACT)
+    return fail(nullptr, MPLBM_ERR_ARG, "bad kernel");
 
   mplbm_ctx* ctx = new mplbm_ctx();
   ctx->cfg = *cfg;
   ctx->ncell = n;
   ctx->bytes_pop = (int)storage_bytes(cfg->storage);
-  ctx->kernel = cfg->kernel == MPLBM_KERNEL_TWOPASS ? MPLBM_KERNEL_TWOPASS : MPLBM_KERNEL_FUSED;
+  ctx->kernel = cfg->kernel == MPLBM_KERNEL_AUTO ? MPLBM_KERNEL_COMPACT : cfg->kernel;
+  ctx->compact = ctx->kernel == MPLBM_KERNEL_COMPACT;
   {
     // sub-plane stride: ly*lz rounded up to 64 elements plus plane_pad
     // elements; a negative plane_pad picks 64 elements when the rounded plane
@@ -1185,7 +1690,7 @@ mplbm_status mplbm_create(const mplbm_config* cfg, mplbm_ctx** out) {
     if (pad < 0) pad = (ps % 4096 == 0) ? 64 : 0;
     ctx->ps = ps + (uint64_t)pad;
   }
-  mplbm_status st = choose_tiling(ctx);
+  mplbm_status st = ctx->compact ? choose_ctiling(ctx) : choose_tiling(ctx);
   if (st != MPLBM_OK) {
     g_create_error = ctx->err;
     delete ctx;
@@ -1207,10 +1712,14 @@ mplbm_status mplbm_create(const mplbm_config* cfg, mplbm_ctx** out) {
   CREATE_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   CREATE_TRY(hipEventCreate(&ctx->ev0));
   CREATE_TRY(hipEventCreate(&ctx->ev1));
-  const size_t lat_bytes = (size_t)cfg->lx * kQ * ctx->ps * ctx->bytes_pop;
+  // compact lattices are sized by the fluid count in mplbm_set_geometry
+  const size_t lat_bytes = ctx->compact ? 0 : (size_t)cfg->lx * kQ * ctx->ps * ctx->bytes_pop;
   const size_t mb = cfg->storage == MPLBM_FP64 ? 8 : 4;  // macro / psi element size
-  CREATE_TRY(hipMalloc(&ctx->lat[0], lat_bytes));
-  CREATE_TRY(hipMalloc(&ctx->lat[1], lat_bytes));
+  if (!ctx->compact) {
+    CREATE_TRY(hipMalloc(&ctx->lat[0], lat_bytes));
+    CREATE_TRY(hipMalloc(&ctx->lat[1], lat_bytes));
+  }
+  ctx->lat_bytes = lat_bytes;
   CREATE_TRY(hipMalloc((void**)&ctx->link, n * 4));
   CREATE_TRY(hipMalloc(&ctx->rho, n * mb));
   CREATE_TRY(hipMalloc(&ctx->pres, n * mb));
@@ -1228,8 +1737,8 @@ mplbm_status mplbm_create(const mplbm_config* cfg, mplbm_ctx** out) {
 void mplbm_destroy(mplbm_ctx* ctx) {
   if (!ctx) return;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  void* bufs[] = {ctx->lat[0], ctx->lat[1], ctx->link, ctx->mask, ctx->rho, ctx->pres,
-                  ctx->u,      ctx->psi,    ctx->red_partial, ctx->red_out};
+  void* bufs[] = {ctx->lat[0], ctx->lat[1], ctx->link, ctx->mask,        ctx->rho,     ctx->pres, ctx->u,
+                  ctx->psi,    ctx->red_partial, ctx->red_out, ctx->rw, ctx->cpos, ctx->ctab};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -1254,8 +1763,14 @@ mplbm_status mplbm_set_geometry(mplbm_ctx* ctx, const uint8_t* is_fluid) {
   HIP_TRY(ctx, hipGetLastError());
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   ctx->fluid_nodes = nf;
-  ctx->have_geometry = true;
+  ctx->have_geometry = false;
   ctx->have_fields = false;
+  if (ctx->compact) {
+    ctx->hmask.swap(m);
+    const mplbm_status st = build_compact(ctx);
+    if (st != MPLBM_OK) return st;
+  }
+  ctx->have_geometry = true;
   return MPLBM_OK;
 }
 
@@ -1303,7 +1818,8 @@ mplbm_status mplbm_init_fields_f64(mplbm_ctx* ctx, const double* rho) {
       auto L = make_lattice<Mo>(ctx, 0, 0);
       hipLaunchKernelGGL(k_init<Mo>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, L,
                          static_cast<typename Mo::S*>(ctx->lat[1]), ph, d_rho);
-      hipLaunchKernelGGL(k_ghost<Mo>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, L);
+      if (!ctx->compact)
+        hipLaunchKernelGGL(k_ghost<Mo>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, L);
       return MPLBM_OK;
     });
     (void)st;
@@ -1444,7 +1960,8 @@ mplbm_status mplbm_write_populations_f64(mplbm_ctx* ctx, const double* f) {
       const auto ph = make_phys<typename Mo::C>(ctx, 0, 1, 0, false);
       auto L = make_lattice<Mo>(ctx, ctx->cur, ctx->cur);
       hipLaunchKernelGGL(k_write_pops<Mo>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, L, ph, d);
-      hipLaunchKernelGGL(k_ghost<Mo>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, L);
+      if (!ctx->compact)
+        hipLaunchKernelGGL(k_ghost<Mo>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, L);
       return MPLBM_OK;
     });
     e = hipGetLastError();
@@ -1497,13 +2014,20 @@ mplbm_status mplbm_set_tiling(mplbm_ctx* ctx, int32_t tile_y, int32_t tile_z, in
   if (!ctx) return fail(ctx, MPLBM_ERR_ARG, "null context");
   const mplbm_config saved = ctx->cfg;
   const Tiling saved_t = ctx->tiling;
+  const CTiling saved_c = ctx->ctiling;
   ctx->cfg.tile_y = tile_y;
   ctx->cfg.tile_z = tile_z;
   ctx->cfg.seg_x = seg_x;
-  mplbm_status st = choose_tiling(ctx);
+  mplbm_status st = ctx->compact ? choose_ctiling(ctx) : choose_tiling(ctx);
+  if (st == MPLBM_OK && ctx->compact && ctx->have_geometry) {
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    st = build_lists(ctx);
+  }
   if (st != MPLBM_OK) {
     ctx->cfg = saved;
     ctx->tiling = saved_t;
+    ctx->ctiling = saved_c;
+    if (ctx->compact && ctx->have_geometry) (void)build_lists(ctx);
   }
   return st;
 }
@@ -1514,9 +2038,9 @@ mplbm_status mplbm_info_get(const mplbm_ctx* ctx, mplbm_info* out) {
   out->cells = (int64_t)ctx->ncell;
   out->storage = ctx->cfg.storage;
   out->kernel = ctx->kernel;
-  out->tile_y = ctx->tiling.TY;
-  out->tile_z = ctx->tiling.TZ;
-  out->seg_x = ctx->tiling.seg;
+  out->tile_y = ctx->compact ? ctx->ctiling.TYi : ctx->tiling.TY;
+  out->tile_z = ctx->compact ? ctx->ctiling.TZi : ctx->tiling.TZ;
+  out->seg_x = ctx->compact ? ctx->ctiling.seg : ctx->tiling.seg;
   out->bytes_per_pop = ctx->bytes_pop;
   out->rho_ref = ctx->rho_ref;
   out->dev_scale = ctx->scale;

@@ -34,7 +34,7 @@ def test_oracle_reproduces_golden(name):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", NAMES)
-@pytest.mark.parametrize("kernel", ["fused", "twopass"])
+@pytest.mark.parametrize("kernel", ["fused", "twopass", "compact"])
 def test_gpu_matches_golden(name, kernel):
     import multiphase_lbm_amd as mp
     g = _load(name)

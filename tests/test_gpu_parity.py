@@ -63,7 +63,7 @@ def _abserr(a, b):
 SC = dict(tau=1.0, G=-5.0, rhos=0.7, rhol_spinodal=1.285931, rhog_spinodal=0.323507)
 
 
-@pytest.mark.parametrize("kernel", ["fused", "twopass"])
+@pytest.mark.parametrize("kernel", ["fused", "twopass", "compact"])
 def test_fp32_small_bit_exact(kernel):
     shape = (12, 10, 14)
     fluid, rho = _case(shape, 3, 1.5, 3.0, seed=3)
@@ -76,7 +76,7 @@ def test_fp32_small_bit_exact(kernel):
     assert np.array_equal(g_p[fluid], st.p[fluid].astype(np.float64))
 
 
-@pytest.mark.parametrize("kernel,bb", [("fused", 0), ("fused", 1), ("twopass", 0)])
+@pytest.mark.parametrize("kernel,bb", [("fused", 0), ("fused", 1), ("twopass", 0), ("compact", 1)])
 def test_fp32_config1_100_steps(kernel, bb):
     """64^3, 40 spheres r in [3,6], G=-5, tau=1, rho0=0.7 +/- 0.01, 100 steps."""
     shape = (64, 64, 64)
@@ -90,11 +90,12 @@ def test_fp32_config1_100_steps(kernel, bb):
     assert stats["nonfinite"] == 0
 
 
-def test_fp64_config1_100_steps():
+@pytest.mark.parametrize("kernel", ["fused", "compact"])
+def test_fp64_config1_100_steps(kernel):
     shape = (64, 64, 64)
     fluid, rho = _case(shape, 40, 3.0, 6.0, seed=1)
-    print("case fp64 config1", flush=True)
-    st, (g_rho, g_p, g_u, g_f, stats) = _run_pair(shape, fluid, rho, 100, "fp64", "fused", **SC)
+    print(f"case fp64 config1 {kernel}", flush=True)
+    st, (g_rho, g_p, g_u, g_f, stats) = _run_pair(shape, fluid, rho, 100, "fp64", kernel, **SC)
     assert _relerr(g_rho[fluid], st.rho[fluid]) <= 1e-12
     umax = float(np.abs(st.u[fluid]).max())
     assert _abserr(g_u[fluid], st.u[fluid]) <= 1e-12 * umax
@@ -118,19 +119,48 @@ def test_fused_tilings_agree(tiles, bb):
     assert np.array_equal(g_f[fluid], st.fA[fluid].astype(np.float64))
 
 
+@pytest.mark.parametrize("tiles", [(1, 1, 1), (3, 5, 2), (8, 32, 3), (0, 0, 0), (2, 19, 9), (23, 7, 4)])
+def test_compact_tilings_agree(tiles):
+    """Compact kernel: single-node tiles, partial tiles in y and z, tiles
+    wider than the domain, short and whole-axis segments."""
+    ty, tz, seg = tiles
+    shape = (9, 23, 19)
+    fluid, rho = _case(shape, 4, 1.5, 3.5, seed=7)
+    print(f"case compact tiling {tiles}", flush=True)
+    st, (g_rho, g_p, g_u, g_f, _) = _run_pair(shape, fluid, rho, 12, "fp32", "compact", tile_y=ty, tile_z=tz,
+                                              seg_x=seg, **SC)
+    assert np.array_equal(g_f[fluid], st.fA[fluid].astype(np.float64))
+    assert np.array_equal(g_rho[fluid], st.rho[fluid].astype(np.float64))
+
+
+@pytest.mark.parametrize("shape", [(5, 2, 3), (2, 3, 40), (1, 6, 5), (3, 1, 1)])
+def test_compact_tiny_domains(shape):
+    """Axes shorter than the tile and its rings (rows and columns of the LDS
+    regions alias the same node), a single plane (both ghost planes are the
+    plane itself)."""
+    rng = np.random.default_rng(sum(shape))
+    fluid = rng.random(shape) < 0.7
+    fluid.flat[0] = True
+    rho = noisy_rho(shape, 0.7, 0.01, 5)
+    print(f"case compact tiny {shape}", flush=True)
+    st, (g_rho, g_p, g_u, g_f, _) = _run_pair(shape, fluid, rho, 6, "fp32", "compact", **SC)
+    assert np.array_equal(g_f[fluid], st.fA[fluid].astype(np.float64))
+
+
+@pytest.mark.parametrize("kernel", ["fused", "compact"])
 @pytest.mark.parametrize("inject_type", [0, 1, 2])
-def test_injection(inject_type):
+def test_injection(inject_type, kernel):
     shape = (16, 20, 12)
     fluid, rho = _case(shape, 3, 2.0, 3.0, seed=11, rho0=1.0, amp=0.3)
-    print(f"case injection type {inject_type}", flush=True)
+    print(f"case injection type {inject_type} {kernel}", flush=True)
     params = dict(SC, rho_inject_period=5, rho_increment=0.005, inject_position=0.5, rhol_spinodal=1.0)
-    st, (g_rho, g_p, g_u, g_f, _) = _run_pair(shape, fluid, rho, 11, "fp32", "fused", step0=0, reverse=-1,
+    st, (g_rho, g_p, g_u, g_f, _) = _run_pair(shape, fluid, rho, 11, "fp32", kernel, step0=0, reverse=-1,
                                               inject_type=inject_type, **params)
     assert np.array_equal(g_f[fluid], st.fA[fluid].astype(np.float64))
     assert np.array_equal(g_rho[fluid], st.rho[fluid].astype(np.float64))
 
 
-@pytest.mark.parametrize("kernel", ["fused", "twopass"])
+@pytest.mark.parametrize("kernel", ["fused", "twopass", "compact"])
 def test_carnahan_starling(kernel):
     shape = (14, 12, 16)
     fluid, rho = _case(shape, 3, 2.0, 3.0, seed=5, rho0=0.2, amp=0.05)
@@ -142,7 +172,7 @@ def test_carnahan_starling(kernel):
     assert np.array_equal(g_f[fluid], st.fA[fluid].astype(np.float64))
 
 
-@pytest.mark.parametrize("kernel", ["fused", "twopass"])
+@pytest.mark.parametrize("kernel", ["fused", "twopass", "compact"])
 def test_pressure_faces_and_body_force(kernel):
     shape = (16, 12, 10)
     fluid, rho = _case(shape, 2, 2.0, 3.0, seed=9, rho0=1.0, amp=0.05)
@@ -155,11 +185,13 @@ def test_pressure_faces_and_body_force(kernel):
     assert np.array_equal(g_u[fluid], st.u[fluid].astype(np.float64))
 
 
-def test_checkpoint_roundtrip_continues_identically():
+@pytest.mark.parametrize("kernel", ["fused", "compact"])
+def test_checkpoint_roundtrip_continues_identically(kernel):
     shape = (12, 12, 12)
     fluid, rho = _case(shape, 2, 2.0, 3.0, seed=13)
-    print("case checkpoint", flush=True)
-    with mp.Context(*shape, storage="fp32", **SC) as a, mp.Context(*shape, storage="fp32", **SC) as b:
+    print(f"case checkpoint {kernel}", flush=True)
+    with mp.Context(*shape, storage="fp32", kernel=kernel, **SC) as a, \
+            mp.Context(*shape, storage="fp32", kernel=kernel, **SC) as b:
         for c in (a, b):
             c.set_geometry(fluid)
             c.init_fields(rho)
