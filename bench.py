@@ -5,7 +5,8 @@ Workload (the packing the headline metric is quoted on): 512^3 domain,
 overlapping spheres with radii uniform in [6, 18] placed by a seeded
 generator until the voxel porosity is 0.45, Shan-Chen G = -5, tau = 1,
 rho0 = 0.7 +/- 0.01 (srand(1) initialisation of the reference).  A step is
-one fused stream-collide launch over the whole lattice.
+one stream-collide launch over the whole lattice (by default the compact
+kernel, which stores and visits fluid nodes only).
 
 Prints one JSON line: value = fluid-node MLUPS of the FP32 form (whole job,
 inputs resident in HBM), the roofline object of the fused kernel (algorithmic
@@ -42,10 +43,11 @@ def parse():
     ap.add_argument("--rmax", type=float, default=18.0)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--storage", default="fp32,fp32fast,fp16s,bf16s")
-    ap.add_argument("--kernel", default="fused")
+    ap.add_argument("--kernel", default="compact", help="compact (fluid nodes only), fused or twopass")
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--tile-z", type=int, default=0)
     ap.add_argument("--seg-x", type=int, default=0)
+    ap.add_argument("--block-threads", type=int, default=0, help="compact kernel workgroup size (0, 256, 512)")
     ap.add_argument("--bounce-back", type=int, default=1, help="0: write-side (ghost) stores, 1: pull-side reads")
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline (0: skip)")
     ap.add_argument("--profile-only", action="store_true", help="run only the FP32 timed loop (for rocprofv3)")
@@ -77,7 +79,7 @@ PHYS = dict(tau=1.0, G=-5.0, rhos=0.7, rhol_spinodal=1.285931, rhog_spinodal=0.3
 def run_gpu(a, shape, fluid, rho, storage):
     import multiphase_lbm_amd as mp
     ctx = mp.Context(*shape, storage=storage, kernel=a.kernel, tile_y=a.tile_y, tile_z=a.tile_z, seg_x=a.seg_x,
-                     bounce_back=a.bounce_back, **PHYS)
+                     bounce_back=a.bounce_back, block_threads=a.block_threads, **PHYS)
     try:
         ctx.set_geometry(fluid)
         ctx.init_fields(rho)
@@ -301,7 +303,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"D3Q19 Shan-Chen step, {shape[0]}x{shape[1]}x{shape[2]} sphere packing "
                                f"(porosity {fluid.mean():.3f}, {nsph} overlapping spheres r~U[{a.rmin},{a.rmax}], "
-                               f"seed {a.seed}), G=-5, tau=1, rho0=0.7+/-0.01, fused {a.kernel} kernel",
+                               f"seed {a.seed}), G=-5, tau=1, rho0=0.7+/-0.01, {a.kernel} stream-collide kernel",
                    "fluid_nodes": r["fluid_nodes"], "tile_y_z_seg": r["tile"],
                    "mlups_unit": "fluid-node updates per second / 1e6"},
         "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

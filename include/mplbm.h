@@ -113,7 +113,8 @@ typedef struct mplbm_config {
   int32_t seg_x;               /* FUSED, COMPACT: planes marched per block (0 = tuned default) */
   int32_t plane_pad;           /* extra elements per population sub-plane (-1 = automatic) */
   int32_t bounce_back;         /* mplbm_bounce_back: how FUSED resolves halfway bounce-back */
-  int32_t reserved2;
+  int32_t block_threads;       /* COMPACT: workgroup size, 256 or 512 (0 = tuned default); interior
+                                  nodes of a plane beyond it are pulled a second time */
 } mplbm_config;
 
 /* FP64 reductions over fluid nodes (mplbm_reduce). */

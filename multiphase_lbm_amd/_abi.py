@@ -37,7 +37,7 @@ class Config(ctypes.Structure):
         ("rho_ref", ctypes.c_double), ("dev_scale", ctypes.c_double),
         ("tile_y", ctypes.c_int32), ("tile_z", ctypes.c_int32),
         ("seg_x", ctypes.c_int32), ("plane_pad", ctypes.c_int32),
-        ("bounce_back", ctypes.c_int32), ("reserved2", ctypes.c_int32),
+        ("bounce_back", ctypes.c_int32), ("block_threads", ctypes.c_int32),
     ]
 
 

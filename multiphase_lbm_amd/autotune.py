@@ -1,10 +1,12 @@
-"""Autotuner of the fused kernel's tile shape.
+"""Autotuner of the marching kernels' tile shape.
 
-Times each candidate (threads in y, threads in z, planes per block) with HIP
-events on the context stream and returns the fastest.  Tiling never changes
-results (each candidate computes the same IEEE operations per node), so the
-tuner may run on the live state; `autotune` re-initialises the fields from
-`rho` afterwards when it is given.
+Times each candidate (tile in y, tile in z, planes per block) with HIP events
+on the context stream and returns the fastest.  For FUSED a tile counts
+threads including the one-cell ring; for COMPACT it counts interior nodes
+(the workgroup is 256 threads up to 256 interior nodes, else 512).  Tiling
+never changes results (each candidate computes the same IEEE operations per
+node), so the tuner may run on the live state; `autotune` re-initialises the
+fields from `rho` afterwards when it is given.
 """
 from __future__ import annotations
 
@@ -26,11 +28,29 @@ def candidates(lx, ly, lz):
     return out
 
 
+def compact_candidates(lx, ly, lz):
+    """Interior tiles of the compact kernel: z spans of 32 or 64 (whole
+    32-node rank words) or the whole axis when it is shorter."""
+    out = []
+    tzs = [lz] if lz <= 32 else [32, 64]
+    for tz, ty in itertools.product(tzs, [4, 8, 12, 16, 24, 32]):
+        if ty > ly or ty * tz > 2048:
+            continue
+        for seg in (16, 32, 64):
+            if seg <= lx:
+                out.append((ty, tz, seg))
+    return out
+
+
 def autotune(ctx, steps=6, rho=None, cands=None, verbose=False):
+    from . import KERNEL_COMPACT
     lx, ly, lz = ctx.shape
     best = None
     results = []
-    for ty, tz, seg in cands or candidates(lx, ly, lz):
+    if cands is None:
+        compact = ctx.info()["kernel"] == KERNEL_COMPACT
+        cands = compact_candidates(lx, ly, lz) if compact else candidates(lx, ly, lz)
+    for ty, tz, seg in cands:
         try:
             ctx.set_tiling(ty, tz, seg)
         except Exception:

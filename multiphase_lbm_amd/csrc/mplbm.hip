@@ -41,6 +41,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "d3q19.hpp"
@@ -812,6 +813,8 @@ __global__ void k_ghost(Lattice<Mo> L) {
 // ----------------------------------------------------------------------------
 // compact marching kernel (fluid nodes only)
 
+constexpr int kCompactSlots = 6;  // LDS slots per region (planes in flight)
+
 struct CTiling {
   int TYi, TZi;          // interior nodes of a tile in y and z
   int PZ, RZ;            // row lengths of the psi region (TZi+2) and the rank region (TZi+4)
@@ -821,7 +824,7 @@ struct CTiling {
   int seg, nseg;         // planes per workgroup, segments along x
   int nblocks;
   int xcd_remap;
-  int threads;           // workgroup size (>= TYi*TZi)
+  int threads;           // workgroup size (256 or 512)
 };
 
 struct CLists {
@@ -841,18 +844,20 @@ struct CLists {
 // registers, (3) barrier, (4) evaluates the force of the fluid interior
 // nodes of plane x from the psi ring, collides them and stores.  Threads take
 // list entries tid, tid + kThreads, ..., so the work follows the fluid count
-// of each plane.  Eight LDS slots per region (plane x' in slot
-// (x' - x0 + 2) & 7) keep every read at least one barrier away from the next
-// write of its slot.
+// of each plane; the workgroup has at least as many threads as the tile has
+// interior nodes.  Six LDS slots per region (plane x' in slot (x' - x0 + 2) mod 6) keep every read at
+// least one barrier away from the next write of its slot: during the pulls
+// of plane x+1 the rank regions of planes x-2 .. x+3 and the psi regions of
+// planes x-2 .. x+3 can be in use.
 template <class Mo, bool kFaces, int kThreads>
 __global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typename Mo::C> ph, CTiling T,
                                                       CLists W) {
   using C = typename Mo::C;
   using S = typename Mo::S;
   extern __shared__ unsigned char smem_raw[];
-  C* psl = reinterpret_cast<C*>(smem_raw);                                   // 8 x PYZ
-  uint32_t* rm = reinterpret_cast<uint32_t*>(smem_raw + 8 * T.PYZ * sizeof(C));  // 8 x RYZ
-  int* ycrd = reinterpret_cast<int*>(rm + 8 * T.RYZ);                        // TYi + 4 rows
+  C* psl = reinterpret_cast<C*>(smem_raw);                                            // kSlots x PYZ
+  uint32_t* rm = reinterpret_cast<uint32_t*>(smem_raw + kCompactSlots * T.PYZ * sizeof(C));  // kSlots x RYZ
+  int* ycrd = reinterpret_cast<int*>(rm + kCompactSlots * T.RYZ);                    // TYi + 4 rows
   int* zcrd = ycrd + (T.TYi + 4);                                            // TZi + 4 columns
 
   int b = blockIdx.x;
@@ -887,23 +892,47 @@ __global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typena
     const int r = v % L.lx;
     return r < 0 ? r + L.lx : r;
   };
-  auto slot = [&](int xp) { return (xp - x0 + 2) & 7; };
+  auto slot = [&](int xp) { return (xp - x0 + 2) % kCompactSlots; };
 
-  // (1) rank region and solid psi of plane xp
-  auto expand = [&](int xp) {
+  // (1) rank region and solid psi of plane xp.  expand_issue loads the
+  // rank words of the first kPre positions of this thread (so their latency
+  // overlaps the pulls that follow), expand_finish writes the region.
+  constexpr int kPre = 2;
+  auto rw_word = [&](const uint2* rwp, int q) {
+    const int r = (int)__umulhi((uint32_t)q, T.magic_rz);
+    const int c = q - r * T.RZ;
+    return rwp[(uint32_t)ycrd[r] * L.W + ((uint32_t)zcrd[c] >> 5)];
+  };
+  auto put_rank = [&](int xp, int q, uint2 w) {
+    const int r = (int)__umulhi((uint32_t)q, T.magic_rz);
+    const int c = q - r * T.RZ;
+    const uint32_t bit = (uint32_t)zcrd[c] & 31u;
+    const bool fl = (w.x >> bit) & 1u;
+    rm[slot(xp) * T.RYZ + q] = fl ? (w.y + __popc(w.x & ((1u << bit) - 1u))) * sz : ~0u;
+    if (!fl && r >= 1 && r <= T.TYi + 2 && c >= 1 && c <= T.TZi + 2)
+      psl[slot(xp) * T.PYZ + (r - 1) * T.PZ + (c - 1)] = psi_solid;
+  };
+  auto expand_issue = [&](int xp, uint2* w) {
     const uint2* rwp = L.rw + (uint64_t)wrapx(xp) * L.ly * L.W;
-    uint32_t* m = rm + slot(xp) * T.RYZ;
-    C* pz = psl + slot(xp) * T.PYZ;
-    for (int q = tid; q < T.RYZ; q += kThreads) {
-      const int r = (int)__umulhi((uint32_t)q, T.magic_rz);
-      const int c = q - r * T.RZ;
-      const int z = zcrd[c];
-      const uint2 w = rwp[(uint32_t)ycrd[r] * L.W + ((uint32_t)z >> 5)];
-      const uint32_t bit = (uint32_t)z & 31u;
-      const bool fl = (w.x >> bit) & 1u;
-      m[q] = fl ? (w.y + __popc(w.x & ((1u << bit) - 1u))) * sz : ~0u;
-      if (!fl && r >= 1 && r <= T.TYi + 2 && c >= 1 && c <= T.TZi + 2) pz[(r - 1) * T.PZ + (c - 1)] = psi_solid;
+#pragma unroll
+    for (int m = 0; m < kPre; ++m) {
+      const int q = tid + m * kThreads;
+      if (q < T.RYZ) w[m] = rw_word(rwp, q);
     }
+  };
+  auto expand_finish = [&](int xp, const uint2* w) {
+#pragma unroll
+    for (int m = 0; m < kPre; ++m) {
+      const int q = tid + m * kThreads;
+      if (q < T.RYZ) put_rank(xp, q, w[m]);
+    }
+    const uint2* rwp = L.rw + (uint64_t)wrapx(xp) * L.ly * L.W;
+    for (int q = tid + kPre * kThreads; q < T.RYZ; q += kThreads) put_rank(xp, q, rw_word(rwp, q));
+  };
+  auto expand = [&](int xp) {
+    uint2 w[kPre];
+    expand_issue(xp, w);
+    expand_finish(xp, w);
   };
 
   // Pull of the fluid node at psi position p of plane xp: populations f,
@@ -956,10 +985,12 @@ __global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typena
   // and position stay in registers (the *N set) until the collision of
   // plane xp one iteration later.  Entries from kThreads on are ring nodes
   // and are pulled first, while only the previous plane's set is live.
-  C fN[kQ], rhoN = (C)0;
+  C fN[kQ];
+  C rhoN = (C)0;
   int pN = 0, gyN = 0, gzN = 0;
   uint32_t ownN = 0;
   bool hasN = false;
+
   auto psi_plane = [&](int xp, bool keep) {
     const uint2 tb = W.tab[(uint64_t)tile * L.lx + wrapx(xp)];
     const uint32_t first = __builtin_amdgcn_readfirstlane(tb.x);
@@ -983,6 +1014,54 @@ __global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typena
     }
   };
 
+  // collision of the fluid interior node at psi position p of plane x
+  auto collide_node = [&](int x, int p, C* f, C rho, int gy, int gz, uint32_t own) {
+    const int xw = wrapx(x);
+    const C* pzm = psl + slot(x - 1) * T.PYZ;
+    const C* pz0 = psl + slot(x) * T.PYZ;
+    const C* pzq = psl + slot(x + 1) * T.PYZ;
+    C F[3];
+    bool skip = false;
+    if constexpr (kFaces) skip = force_skipped(ph, xw, gy, gz, L.lx, L.ly, L.lz);
+    if (skip) {
+      F[0] = F[1] = F[2] = (C)0;
+    } else {
+      C pn[kQ];
+      pn[0] = (C)0;
+#pragma unroll
+      for (int i = 1; i < kQ; ++i) {
+        const C* pp = ex(i) == 1 ? pzq : (ex(i) == -1 ? pzm : pz0);
+        pn[i] = pp[p + ey(i) * T.PZ + ez(i)];
+      }
+      sc_force(pn, pz0[p], ph, F);
+    }
+    C u[3];
+    collide<Mo>(f, rho, F, ph, u);
+    const Rsrc O = make_rsrc(L.fout + (uint64_t)(xw + 1) * kQ * L.ps);
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) bstore<S>(encode<Mo>(f[i], i, ph), O, own, (uint32_t)i * psb);
+    if (xw == 0 || xw == L.lx - 1) {  // ghost copies of the first and last planes
+      if (xw == 0) {
+        const Rsrc G = make_rsrc(L.fout + (uint64_t)(L.lx + 1) * kQ * L.ps);
+#pragma unroll
+        for (int i = 0; i < kQ; ++i) bstore<S>(encode<Mo>(f[i], i, ph), G, own, (uint32_t)i * psb);
+      }
+      if (xw == L.lx - 1) {
+        const Rsrc G = make_rsrc(L.fout);
+#pragma unroll
+        for (int i = 0; i < kQ; ++i) bstore<S>(encode<Mo>(f[i], i, ph), G, own, (uint32_t)i * psb);
+      }
+    }
+    if (ph.write_macro) {
+      const uint64_t self = ((uint64_t)xw * L.ly + gy) * L.lz + gz;
+      L.rho[self] = (typename Mo::M)rho;
+      L.pres[self] = (typename Mo::M)pressure_of<Mo>(ph, rho);
+      L.u[self] = (typename Mo::M)u[0];
+      L.u[(uint64_t)L.ncell + self] = (typename Mo::M)u[1];
+      L.u[2ull * L.ncell + self] = (typename Mo::M)u[2];
+    }
+  };
+
   // prologue: rank regions of planes x0-2 .. x0+2, psi of planes x0-1, x0
   for (int k = -2; k <= 2; ++k) expand(x0 + k);
   __syncthreads();
@@ -991,7 +1070,10 @@ __global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typena
 
   for (int j = 0; j < n; ++j) {
     const int x = __builtin_amdgcn_readfirstlane(x0 + j);
-    // this thread's interior node of plane x
+    const bool more = j + 2 <= n;  // rank regions are needed up to plane x1 + 1
+    uint2 wpre[kPre];
+    if (more) expand_issue(x + 3, wpre);
+    // this thread's interior node of plane x, pulled one iteration ago
     C f[kQ];
 #pragma unroll
     for (int i = 0; i < kQ; ++i) f[i] = fN[i];
@@ -999,58 +1081,12 @@ __global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typena
     const int p = pN, gy = gyN, gz = gzN;
     const uint32_t own = ownN;
     const bool has = hasN;
-
-    if (j + 2 <= n) expand(x + 3);  // needed up to plane x1 + 1
     psi_plane(x + 1, true);
+    if (more) expand_finish(x + 3, wpre);
     __syncthreads();
 
     // (4) collide this thread's fluid interior node of plane x
-    if (has) {
-      const int xw = wrapx(x);
-      const C* pzm = psl + slot(x - 1) * T.PYZ;
-      const C* pz0 = psl + slot(x) * T.PYZ;
-      const C* pzq = psl + slot(x + 1) * T.PYZ;
-      C F[3];
-      bool skip = false;
-      if constexpr (kFaces) skip = force_skipped(ph, xw, gy, gz, L.lx, L.ly, L.lz);
-      if (skip) {
-        F[0] = F[1] = F[2] = (C)0;
-      } else {
-        C pn[kQ];
-        pn[0] = (C)0;
-#pragma unroll
-        for (int i = 1; i < kQ; ++i) {
-          const C* pp = ex(i) == 1 ? pzq : (ex(i) == -1 ? pzm : pz0);
-          pn[i] = pp[p + ey(i) * T.PZ + ez(i)];
-        }
-        sc_force(pn, pz0[p], ph, F);
-      }
-      C u[3];
-      collide<Mo>(f, rho, F, ph, u);
-      const Rsrc O = make_rsrc(L.fout + (uint64_t)(xw + 1) * kQ * L.ps);
-#pragma unroll
-      for (int i = 0; i < kQ; ++i) bstore<S>(encode<Mo>(f[i], i, ph), O, own, (uint32_t)i * psb);
-      if (xw == 0 || xw == L.lx - 1) {  // ghost copies of the first and last planes
-        if (xw == 0) {
-          const Rsrc G = make_rsrc(L.fout + (uint64_t)(L.lx + 1) * kQ * L.ps);
-#pragma unroll
-          for (int i = 0; i < kQ; ++i) bstore<S>(encode<Mo>(f[i], i, ph), G, own, (uint32_t)i * psb);
-        }
-        if (xw == L.lx - 1) {
-          const Rsrc G = make_rsrc(L.fout);
-#pragma unroll
-          for (int i = 0; i < kQ; ++i) bstore<S>(encode<Mo>(f[i], i, ph), G, own, (uint32_t)i * psb);
-        }
-      }
-      if (ph.write_macro) {
-        const uint64_t self = ((uint64_t)xw * L.ly + gy) * L.lz + gz;
-        L.rho[self] = (typename Mo::M)rho;
-        L.pres[self] = (typename Mo::M)pressure_of<Mo>(ph, rho);
-        L.u[self] = (typename Mo::M)u[0];
-        L.u[(uint64_t)L.ncell + self] = (typename Mo::M)u[1];
-        L.u[2ull * L.ncell + self] = (typename Mo::M)u[2];
-      }
-    }
+    if (has) collide_node(x, p, f, rho, gy, gz, own);
   }
 }
 
@@ -1403,7 +1439,6 @@ static int fused_threads(const Tiling& T) { return ((T.TY * T.TZ + 63) / 64) * 6
 // Tile of the compact kernel: TZi x TYi interior nodes (z a multiple of 32
 // where the axis allows, so a tile row covers whole rank words), segments of
 // seg planes; about 1024 workgroups or more.
-constexpr int kCompactMaxThreads = 512;  // workgroup size >= interior nodes of a tile
 
 static mplbm_status choose_ctiling(mplbm_ctx* ctx) {
   const mplbm_config& c = ctx->cfg;
@@ -1414,8 +1449,7 @@ static mplbm_status choose_ctiling(mplbm_ctx* ctx) {
   if (tz > c.lz) tz = (int)c.lz;
   if (ty > c.ly) ty = (int)c.ly;
   if (tz < 1 || ty < 1) return fail(ctx, MPLBM_ERR_ARG, "tile %dx%d too small", ty, tz);
-  if ((int64_t)ty * tz > kCompactMaxThreads)
-    return fail(ctx, MPLBM_ERR_ARG, "tile %dx%d exceeds %d interior nodes", ty, tz, kCompactMaxThreads);
+  if ((int64_t)(ty + 2) * (tz + 2) >= 65536) return fail(ctx, MPLBM_ERR_ARG, "tile %dx%d too large", ty, tz);
   T.TYi = ty;
   T.TZi = tz;
   T.PZ = tz + 2;
@@ -1428,7 +1462,7 @@ static mplbm_status choose_ctiling(mplbm_ctx* ctx) {
   T.tiles_z = (int)((c.lz + tz - 1) / tz);
   {
     const size_t csz = c.storage == MPLBM_FP64 ? 8 : 4;
-    const size_t lds = 8 * (size_t)T.PYZ * csz + 8 * (size_t)T.RYZ * 4 + (size_t)(ty + 4 + tz + 4) * 4;
+    const size_t lds = kCompactSlots * ((size_t)T.PYZ * csz + (size_t)T.RYZ * 4) + (size_t)(ty + 4 + tz + 4) * 4;
     if (lds > 65536) return fail(ctx, MPLBM_ERR_ARG, "tile %dx%d needs %zu bytes of LDS (max 65536)", ty, tz, lds);
   }
   const int64_t tiles = (int64_t)T.tiles_y * T.tiles_z;
@@ -1443,14 +1477,26 @@ static mplbm_status choose_ctiling(mplbm_ctx* ctx) {
   if (nb > (1LL << 30)) return fail(ctx, MPLBM_ERR_ARG, "too many blocks");
   T.nblocks = (int)nb;
   T.xcd_remap = 1;
-  T.threads = ty * tz <= 256 ? 256 : kCompactMaxThreads;
+  if (c.block_threads != 0 && c.block_threads != 256 && c.block_threads != 512 && c.block_threads != 1024)
+    return fail(ctx, MPLBM_ERR_ARG, "block_threads must be 0, 256, 512 or 1024");
+  {
+    const int area = ty * tz;
+    const int need = area <= 256 ? 256 : (area <= 512 ? 512 : 1024);
+    const int maxt = c.storage == MPLBM_FP64 ? 512 : 1024;  // FP64 needs > 128 VGPRs
+    if (area > maxt)
+      return fail(ctx, MPLBM_ERR_ARG, "tile %dx%d exceeds %d interior nodes for this storage", ty, tz, maxt);
+    if (c.block_threads && c.block_threads < area)
+      return fail(ctx, MPLBM_ERR_ARG, "block_threads %d below the tile's %d interior nodes", c.block_threads, area);
+    T.threads = c.block_threads ? c.block_threads : need;
+    if (T.threads > maxt) return fail(ctx, MPLBM_ERR_ARG, "block_threads %d too large for this storage", T.threads);
+  }
   ctx->ctiling = T;
   return MPLBM_OK;
 }
 
 template <class C>
 static size_t compact_lds_bytes(const CTiling& T) {
-  return 8 * (size_t)T.PYZ * sizeof(C) + 8 * (size_t)T.RYZ * 4 + (size_t)(T.TYi + 4 + T.TZi + 4) * 4;
+  return kCompactSlots * ((size_t)T.PYZ * sizeof(C) + (size_t)T.RYZ * 4) + (size_t)(T.TYi + 4 + T.TZi + 4) * 4;
 }
 
 // Work lists of the compact kernel: for every tile and plane, the psi-region
@@ -1570,16 +1616,23 @@ static mplbm_status launch_step(mplbm_ctx* ctx, int64_t step, int32_t reverse, u
     const int grid = T.xcd_remap ? ((T.nblocks + 7) / 8) * 8 : T.nblocks;
     const size_t shm = compact_lds_bytes<C>(T);
     const CLists WL{ctx->cpos, ctx->ctab};
+    // the workgroup size is at least the tile's interior count (one kept
+    // interior node per thread); 1024 only for non-FP64 storage
     if (T.threads == 256) {
       if (ph.any_face)
         hipLaunchKernelGGL((k_compact<Mo, true, 256>), dim3(grid), dim3(256), shm, ctx->stream, L, ph, T, WL);
       else
         hipLaunchKernelGGL((k_compact<Mo, false, 256>), dim3(grid), dim3(256), shm, ctx->stream, L, ph, T, WL);
-    } else {
+    } else if (T.threads == 512) {
       if (ph.any_face)
         hipLaunchKernelGGL((k_compact<Mo, true, 512>), dim3(grid), dim3(512), shm, ctx->stream, L, ph, T, WL);
       else
         hipLaunchKernelGGL((k_compact<Mo, false, 512>), dim3(grid), dim3(512), shm, ctx->stream, L, ph, T, WL);
+    } else if constexpr (Mo::kStorage != MPLBM_FP64) {
+      if (ph.any_face)
+        hipLaunchKernelGGL((k_compact<Mo, true, 1024>), dim3(grid), dim3(1024), shm, ctx->stream, L, ph, T, WL);
+      else
+        hipLaunchKernelGGL((k_compact<Mo, false, 1024>), dim3(grid), dim3(1024), shm, ctx->stream, L, ph, T, WL);
     }
   } else if (ctx->kernel == MPLBM_KERNEL_FUSED) {
     const Tiling& T = ctx->tiling;

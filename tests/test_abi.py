@@ -67,3 +67,15 @@ def test_create_reports_errors_without_exit():
         mp.Context(8, 8, 8, face_type=(1, 0, 0, 0, 0, 0))
     with pytest.raises(mp.MplbmError, match="2\\^32"):
         mp.Context(2048, 2048, 2048)
+
+
+def test_compact_tile_limits_report_errors():
+    """A compact workgroup holds one interior node per thread; tiles beyond
+    that or beyond the LDS budget are refused before any device work."""
+    import multiphase_lbm_amd as mp
+    with pytest.raises(mp.MplbmError, match="below the tile"):
+        mp.Context(64, 64, 64, kernel="compact", tile_y=16, tile_z=32, block_threads=256)
+    with pytest.raises(mp.MplbmError, match="block_threads must be"):
+        mp.Context(64, 64, 64, kernel="compact", block_threads=300)
+    with pytest.raises(mp.MplbmError, match="exceeds 512 interior nodes"):
+        mp.Context(64, 64, 64, storage="fp64", kernel="compact", tile_y=24, tile_z=24)

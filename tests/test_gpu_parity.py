@@ -42,7 +42,7 @@ def _run_pair(shape, fluid, rho, steps, storage="fp32", kernel="fused", step0=0,
     return st, (g_rho, g_p, g_u, g_f, stats)
 
 
-LIB_ONLY = ("tile_y", "tile_z", "seg_x", "rho_ref", "dev_scale", "bounce_back", "plane_pad")
+LIB_ONLY = ("tile_y", "tile_z", "seg_x", "rho_ref", "dev_scale", "bounce_back", "plane_pad", "block_threads")
 
 
 def _lib_params(params):
@@ -119,16 +119,18 @@ def test_fused_tilings_agree(tiles, bb):
     assert np.array_equal(g_f[fluid], st.fA[fluid].astype(np.float64))
 
 
-@pytest.mark.parametrize("tiles", [(1, 1, 1), (3, 5, 2), (8, 32, 3), (0, 0, 0), (2, 19, 9), (23, 7, 4)])
+@pytest.mark.parametrize("tiles", [(1, 1, 1, 0), (3, 5, 2, 0), (8, 32, 3, 0), (0, 0, 0, 0), (2, 19, 9, 512),
+                                   (23, 7, 4, 0), (16, 32, 5, 0), (23, 19, 9, 1024)])
 def test_compact_tilings_agree(tiles):
     """Compact kernel: single-node tiles, partial tiles in y and z, tiles
-    wider than the domain, short and whole-axis segments."""
-    ty, tz, seg = tiles
+    wider than the domain, short and whole-axis segments, workgroups of 256,
+    512 and 1024 threads."""
+    ty, tz, seg, nt = tiles
     shape = (9, 23, 19)
     fluid, rho = _case(shape, 4, 1.5, 3.5, seed=7)
     print(f"case compact tiling {tiles}", flush=True)
     st, (g_rho, g_p, g_u, g_f, _) = _run_pair(shape, fluid, rho, 12, "fp32", "compact", tile_y=ty, tile_z=tz,
-                                              seg_x=seg, **SC)
+                                              seg_x=seg, block_threads=nt, **SC)
     assert np.array_equal(g_f[fluid], st.fA[fluid].astype(np.float64))
     assert np.array_equal(g_rho[fluid], st.rho[fluid].astype(np.float64))
 
