@@ -859,6 +859,8 @@ __global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typena
   uint32_t* rm = reinterpret_cast<uint32_t*>(smem_raw + kCompactSlots * T.PYZ * sizeof(C));  // kSlots x RYZ
   int* ycrd = reinterpret_cast<int*>(rm + kCompactSlots * T.RYZ);                    // TYi + 4 rows
   int* zcrd = ycrd + (T.TYi + 4);                                            // TZi + 4 columns
+  uint32_t* qtab = reinterpret_cast<uint32_t*>(zcrd + (T.TZi + 4));          // RYZ: rank word << 5 | bit
+  int* qpsi = reinterpret_cast<int*>(qtab + T.RYZ);                          // RYZ: psi index or -1
 
   int b = blockIdx.x;
   if (T.xcd_remap) {
@@ -886,11 +888,22 @@ __global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typena
   for (int q = tid; q < T.TYi + 4; q += kThreads) ycrd[q] = ((y0 - 2 + q) % L.ly + L.ly) % L.ly;
   for (int q = tid; q < T.TZi + 4; q += kThreads) zcrd[q] = ((z0 - 2 + q) % L.lz + L.lz) % L.lz;
   __syncthreads();
+  // per rank-region position: its rank word within a plane's words, the bit,
+  // and its index in the psi region (the same for every plane)
+  for (int q = tid; q < T.RYZ; q += kThreads) {
+    const int r = (int)__umulhi((uint32_t)q, T.magic_rz);
+    const int c = q - r * T.RZ;
+    const uint32_t z = (uint32_t)zcrd[c];
+    qtab[q] = (((uint32_t)ycrd[r] * L.W + (z >> 5)) << 5) | (z & 31u);
+    qpsi[q] = (r >= 1 && r <= T.TYi + 2 && c >= 1 && c <= T.TZi + 2) ? (r - 1) * T.PZ + (c - 1) : -1;
+  }
+  __syncthreads();
 
-  // planes x0-2 .. x1+1 are addressed; with lx < 3 one wrap is not enough
+  // planes x0-2 .. x1+1 are addressed: one wrap suffices for lx >= 2, and
+  // every plane is plane 0 when lx = 1
   auto wrapx = [&](int v) {
-    const int r = v % L.lx;
-    return r < 0 ? r + L.lx : r;
+    if (L.lx == 1) return 0;
+    return v >= L.lx ? v - L.lx : (v < 0 ? v + L.lx : v);
   };
   auto slot = [&](int xp) { return (xp - x0 + 2) % kCompactSlots; };
 
@@ -898,19 +911,15 @@ __global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typena
   // rank words of the first kPre positions of this thread (so their latency
   // overlaps the pulls that follow), expand_finish writes the region.
   constexpr int kPre = 2;
-  auto rw_word = [&](const uint2* rwp, int q) {
-    const int r = (int)__umulhi((uint32_t)q, T.magic_rz);
-    const int c = q - r * T.RZ;
-    return rwp[(uint32_t)ycrd[r] * L.W + ((uint32_t)zcrd[c] >> 5)];
-  };
+  auto rw_word = [&](const uint2* rwp, int q) { return rwp[qtab[q] >> 5]; };
   auto put_rank = [&](int xp, int q, uint2 w) {
-    const int r = (int)__umulhi((uint32_t)q, T.magic_rz);
-    const int c = q - r * T.RZ;
-    const uint32_t bit = (uint32_t)zcrd[c] & 31u;
+    const uint32_t bit = qtab[q] & 31u;
     const bool fl = (w.x >> bit) & 1u;
     rm[slot(xp) * T.RYZ + q] = fl ? (w.y + __popc(w.x & ((1u << bit) - 1u))) * sz : ~0u;
-    if (!fl && r >= 1 && r <= T.TYi + 2 && c >= 1 && c <= T.TZi + 2)
-      psl[slot(xp) * T.PYZ + (r - 1) * T.PZ + (c - 1)] = psi_solid;
+    if (!fl) {
+      const int pi = qpsi[q];
+      if (pi >= 0) psl[slot(xp) * T.PYZ + pi] = psi_solid;
+    }
   };
   auto expand_issue = [&](int xp, uint2* w) {
     const uint2* rwp = L.rw + (uint64_t)wrapx(xp) * L.ly * L.W;
@@ -987,7 +996,7 @@ __global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typena
   // and are pulled first, while only the previous plane's set is live.
   C fN[kQ];
   C rhoN = (C)0;
-  int pN = 0, gyN = 0, gzN = 0;
+  int pN = 0;
   uint32_t ownN = 0;
   bool hasN = false;
 
@@ -1008,7 +1017,8 @@ __global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typena
     hasN = false;
     if (tid < npull) {
       pN = W.pos[first + tid];
-      rhoN = pull(xp, pN, fN, gyN, gzN, ownN);
+      int gy, gz;
+      rhoN = pull(xp, pN, fN, gy, gz, ownN);
       pz[pN] = psi_of<Mo>(ph, rhoN);
       hasN = keep && tid < nint;
     }
@@ -1078,7 +1088,7 @@ __global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typena
 #pragma unroll
     for (int i = 0; i < kQ; ++i) f[i] = fN[i];
     const C rho = rhoN;
-    const int p = pN, gy = gyN, gz = gzN;
+    const int p = pN;
     const uint32_t own = ownN;
     const bool has = hasN;
     psi_plane(x + 1, true);
@@ -1086,7 +1096,10 @@ __global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typena
     __syncthreads();
 
     // (4) collide this thread's fluid interior node of plane x
-    if (has) collide_node(x, p, f, rho, gy, gz, own);
+    if (has) {
+      const int r = (int)__umulhi((uint32_t)p, T.magic_pz);
+      collide_node(x, p, f, rho, ycrd[r + 1], zcrd[p - r * T.PZ + 1], own);
+    }
   }
 }
 
@@ -1444,8 +1457,11 @@ static mplbm_status choose_ctiling(mplbm_ctx* ctx) {
   const mplbm_config& c = ctx->cfg;
   CTiling T{};
   int tz = c.tile_z, ty = c.tile_y, seg = c.seg_x;
+  // measured on 512^3 packings: 8 x 32 interior nodes (256 threads) for
+  // 16-bit storage, 16 x 32 (512 threads) for FP32 and FP64
+  const int ty_def = (c.storage == MPLBM_FP16S || c.storage == MPLBM_BF16S) ? 8 : 16;
   if (tz <= 0) tz = c.lz <= 32 ? (int)c.lz : 32;
-  if (ty <= 0) ty = c.ly <= 8 ? (int)c.ly : 8;
+  if (ty <= 0) ty = c.ly <= ty_def ? (int)c.ly : ty_def;
   if (tz > c.lz) tz = (int)c.lz;
   if (ty > c.ly) ty = (int)c.ly;
   if (tz < 1 || ty < 1) return fail(ctx, MPLBM_ERR_ARG, "tile %dx%d too small", ty, tz);
@@ -1462,12 +1478,13 @@ static mplbm_status choose_ctiling(mplbm_ctx* ctx) {
   T.tiles_z = (int)((c.lz + tz - 1) / tz);
   {
     const size_t csz = c.storage == MPLBM_FP64 ? 8 : 4;
-    const size_t lds = kCompactSlots * ((size_t)T.PYZ * csz + (size_t)T.RYZ * 4) + (size_t)(ty + 4 + tz + 4) * 4;
+    const size_t lds = kCompactSlots * ((size_t)T.PYZ * csz + (size_t)T.RYZ * 4) + (size_t)(ty + 4 + tz + 4) * 4 +
+                       2 * (size_t)T.RYZ * 4;
     if (lds > 65536) return fail(ctx, MPLBM_ERR_ARG, "tile %dx%d needs %zu bytes of LDS (max 65536)", ty, tz, lds);
   }
   const int64_t tiles = (int64_t)T.tiles_y * T.tiles_z;
   if (seg <= 0) {
-    seg = 64;
+    seg = 32;
     while (seg > 4 && tiles * ((c.lx + seg - 1) / seg) < 2048) seg /= 2;
   }
   if (seg > c.lx) seg = (int)c.lx;
@@ -1496,7 +1513,8 @@ static mplbm_status choose_ctiling(mplbm_ctx* ctx) {
 
 template <class C>
 static size_t compact_lds_bytes(const CTiling& T) {
-  return kCompactSlots * ((size_t)T.PYZ * sizeof(C) + (size_t)T.RYZ * 4) + (size_t)(T.TYi + 4 + T.TZi + 4) * 4;
+  return kCompactSlots * ((size_t)T.PYZ * sizeof(C) + (size_t)T.RYZ * 4) + (size_t)(T.TYi + 4 + T.TZi + 4) * 4 +
+         2 * (size_t)T.RYZ * 4;
 }
 
 // Work lists of the compact kernel: for every tile and plane, the psi-region
