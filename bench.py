@@ -139,8 +139,8 @@ def traffic_from_profiles(storage, tag):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE +
     WRITE_SIZE, each scaled by the copy-kernel calibration) for this
     workload and kernel configuration, from the `traffic` lines of
-    profiles/traffic_fused_512.txt; None when not measured."""
-    path = os.path.join(ROOT, "profiles", "traffic_fused_512.txt")
+    profiles/traffic_512.txt; None when not measured."""
+    path = os.path.join(ROOT, "profiles", "traffic_512.txt")
     try:
         with open(path) as f:
             for line in f:
@@ -162,7 +162,7 @@ def precision_study(a, shape, fluid, rho):
     fl = fluid.astype(bool)
 
     def run(storage, params, steps, rho0, **kw):
-        with mp.Context(*shape, storage=storage, kernel="fused", **params, **kw) as ctx:
+        with mp.Context(*shape, storage=storage, kernel=a.kernel, **params, **kw) as ctx:
             ctx.set_geometry(fluid)
             ctx.init_fields(rho0)
             t = time.perf_counter()
@@ -199,7 +199,7 @@ def precision_study(a, shape, fluid, rho):
     sfl = np.ones(sshape, np.uint8)
 
     def run_slab(storage, **kw):
-        with mp.Context(*sshape, storage=storage, kernel="fused", tau=1.0, G=-5.0, rhos=0.7, **kw) as ctx:
+        with mp.Context(*sshape, storage=storage, kernel=a.kernel, tau=1.0, G=-5.0, rhos=0.7, **kw) as ctx:
             ctx.set_geometry(sfl)
             ctx.init_fields(slab.astype(np.float64 if storage == "fp64" else np.float32))
             ctx.step(0, a.slab_steps, write_macro=True)

@@ -681,7 +681,10 @@ bool MultiphaseLBM::mplbm_parse_scalar_token(std::istream& file, const std::stri
     in.math = s == "fast" ? MPLBM_MATH_FAST : MPLBM_MATH_REFERENCE;
   } else if (token == "kernel") {
     file >> s;
-    in.kernel = s == "twopass" ? MPLBM_KERNEL_TWOPASS : s == "fused" ? MPLBM_KERNEL_FUSED : MPLBM_KERNEL_AUTO;
+    in.kernel = s == "twopass"   ? MPLBM_KERNEL_TWOPASS
+                : s == "fused"   ? MPLBM_KERNEL_FUSED
+                : s == "compact" ? MPLBM_KERNEL_COMPACT
+                                 : MPLBM_KERNEL_AUTO;
   } else if (token == "pressure_x" || token == "pressure_y" || token == "pressure_z") {
     const int a = token[9] - 'x';
     double lo, hi;
