@@ -960,16 +960,23 @@ __global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typena
     const uint32_t* mm = rm + slot(xp - 1) * T.RYZ + q0;
     const uint32_t* mq = rm + slot(xp + 1) * T.RYZ + q0;
     self = m0[0];
-    const uint32_t own = self + plb;  // own node, relative to memory plane xw
+    // psb is made opaque here so the 38 multiples of it are formed at their
+    // use instead of being kept live in scalar registers across the loop
+    uint32_t psb_o = psb;
+    asm volatile("" : "+s"(psb_o));
+    const uint32_t own = self + kQ * psb_o;  // own node, relative to memory plane xw
     const Rsrc R = make_rsrc(L.fin + (uint64_t)xw * kQ * L.ps);
     S raw[kQ];
 #pragma unroll
     for (int i = 0; i < kQ; ++i) {
       const uint32_t* mi = ex(i) == 1 ? mm : (ex(i) == -1 ? mq : m0);
       const uint32_t sr = mi[-ey(i) * T.RZ - ez(i)];
-      const uint32_t voff = sr != ~0u ? sr + (uint32_t)(1 - ex(i)) * plb + (uint32_t)i * psb
-                                      : own + (uint32_t)opp(i) * psb;
-      raw[i] = bload<S>(R, voff, 0);
+      // source in plane xp - e_x(i): sub-plane (1 - e_x)*19 + i from memory
+      // plane xw; a solid source: own node, sub-plane 19 + opp(i)
+      const bool fl = sr != ~0u;
+      const uint32_t base = fl ? sr : self;
+      const uint32_t k = fl ? (uint32_t)((1 - ex(i)) * kQ + i) : (uint32_t)(kQ + opp(i));
+      raw[i] = bload<S>(R, base + __umul24(k, psb_o), 0);
     }
 #pragma unroll
     for (int i = 0; i < kQ; ++i) f[i] = decode<Mo>(raw[i], i, ph);
@@ -1001,7 +1008,10 @@ __global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typena
   bool hasN = false;
 
   auto psi_plane = [&](int xp, bool keep) {
-    const uint2 tb = W.tab[(uint64_t)tile * L.lx + wrapx(xp)];
+    // uniform index: a scalar load through the constant address space
+    using CU64 = const __attribute__((address_space(4))) unsigned long long;
+    const unsigned long long tw = ((CU64*)W.tab)[(uint64_t)tile * L.lx + wrapx(xp)];
+    const uint2 tb = make_uint2((uint32_t)tw, (uint32_t)(tw >> 32));
     const uint32_t first = __builtin_amdgcn_readfirstlane(tb.x);
     const int nint = (int)__builtin_amdgcn_readfirstlane(tb.y & 0xFFFFu);
     const int npull = (int)__builtin_amdgcn_readfirstlane(tb.y >> 16);
@@ -1048,8 +1058,10 @@ __global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typena
     C u[3];
     collide<Mo>(f, rho, F, ph, u);
     const Rsrc O = make_rsrc(L.fout + (uint64_t)(xw + 1) * kQ * L.ps);
+    uint32_t psb_o = psb;
+    asm volatile("" : "+s"(psb_o));
 #pragma unroll
-    for (int i = 0; i < kQ; ++i) bstore<S>(encode<Mo>(f[i], i, ph), O, own, (uint32_t)i * psb);
+    for (int i = 0; i < kQ; ++i) bstore<S>(encode<Mo>(f[i], i, ph), O, own, (uint32_t)i * psb_o);
     if (xw == 0 || xw == L.lx - 1) {  // ghost copies of the first and last planes
       if (xw == 0) {
         const Rsrc G = make_rsrc(L.fout + (uint64_t)(L.lx + 1) * kQ * L.ps);
