@@ -57,6 +57,10 @@ def parse():
     ap.add_argument("--perm-steps", type=int, default=3000)
     ap.add_argument("--sweep", action="store_true", help="time every autotuner tile candidate (FP32) and exit")
     ap.add_argument("--autotune", action="store_true", help="pick the tile with the autotuner before timing")
+    ap.add_argument("--ensemble", action="store_true",
+                    help="batched ensemble: 64 packings n^3, n~U{48..160}, one persistent launch per step")
+    ap.add_argument("--ensemble-samples", type=int, default=64)
+    ap.add_argument("--ensemble-steps", type=int, default=5000, help="steps before the saturation read-out")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain device-to-device copy kernel of 2 GiB (rocprofv3 --pmc calibration)")
     return ap.parse_args()
@@ -253,6 +257,61 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}), flush=True)
 
 
+def ensemble_bench(a):
+    """Batched ensemble: 64 seeded packings, cubic sizes n ~ U{48..160},
+    porosity ~ U[0.35, 0.55], radii ~ U[2, 8], G ~ U[-5.5, -4.5] (seeds
+    0..63), all advanced by one persistent launch per step (hipGraph replay),
+    FP32 and FP16 storage, workgroup size picked by the autotuner.  Prints one
+    JSON line with the aggregate MLUPS and per-sample saturation."""
+    from multiphase_lbm_amd import harness as H
+    from multiphase_lbm_amd.ensemble import Ensemble, autotune_block
+    rng = np.random.default_rng(2024)
+    samples, fluids, rhos = [], [], []
+    t = time.time()
+    for k in range(a.ensemble_samples):
+        n = int(rng.integers(48, 161))
+        por = float(rng.uniform(0.35, 0.55))
+        G = float(rng.uniform(-5.5, -4.5))
+        fl, _ = H.generate_packing((n, n, n), k, 10 ** 7, 2.0, 8.0, por)
+        samples.append(dict(lx=n, ly=n, lz=n, G=G, tau=1.0, rhos=0.7, rhol_spinodal=1.285931,
+                            rhog_spinodal=0.323507))
+        fluids.append(fl)
+        rhos.append(H.rho_init((n, n, n), 0.7, 0.02 / 0.7))
+    nf = int(sum(int(f.sum()) for f in fluids))
+    cells = int(sum(f.size for f in fluids))
+    print(f"# ensemble: {len(samples)} samples, {cells} cells, {nf} fluid nodes, built in {time.time() - t:.1f}s",
+          file=sys.stderr, flush=True)
+    out = {"workload": f"{len(samples)} packings n^3, n~U{{48..160}}, porosity~U[0.35,0.55], r~U[2,8], "
+                       f"G~U[-5.5,-4.5], seeds 0..{len(samples) - 1}",
+           "cells": cells, "fluid_nodes": nf}
+    for storage in a.storage.split(","):
+        def make(block):
+            e = Ensemble(samples, storage=storage, block_threads=block, kernel=a.kernel)
+            for k in range(len(samples)):
+                e.set_geometry(k, fluids[k])
+                e.init_fields(k, rhos[k])
+            return e
+        best, times = autotune_block(make)
+        with make(best) as ens:
+            ens.step(0, a.warmup)
+            ms = ens.step_timed(a.warmup, a.steps) / a.steps
+            done = a.warmup + a.steps
+            rest = max(1, a.ensemble_steps - done)
+            ens.step(done, rest, write_macro=True)
+            sat = []
+            for k in range(len(samples)):
+                r, _, _ = ens.read_fields(k)
+                fl = fluids[k].astype(bool)
+                sat.append(round(float((r[fl] > 0.5 * (1.932442 + 0.156413)).mean()), 4))
+            summ = ens.summary()
+        out[storage] = {"block_threads": best, "block_ms": {str(b): round(v, 4) for b, v in times.items()},
+                        "ms_per_step": round(ms, 4), "mlups": round(nf / ms / 1e3, 1),
+                        "work_items": summ["work_items"], "saturation_step": done + rest, "saturation": sat}
+        print(f"# ensemble {storage}: block {best}, {ms:.3f} ms/step, {nf / ms / 1e3:.1f} MLUPS", file=sys.stderr,
+              flush=True)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -262,6 +321,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU slab decomposition is "
               "a design note, not built", file=sys.stderr)
         sys.exit(2)
+    if a.ensemble:
+        ensemble_bench(a)
+        return
     shape, fluid, rho, nsph = workload(a)
     if a.precision:
         precision_study(a, shape, fluid, rho)

@@ -850,11 +850,10 @@ struct CLists {
 // of plane x+1 the rank regions of planes x-2 .. x+3 and the psi regions of
 // planes x-2 .. x+3 can be in use.
 template <class Mo, bool kFaces, int kThreads>
-__global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typename Mo::C> ph, CTiling T,
-                                                      CLists W) {
+__device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<typename Mo::C>& ph, const CTiling& T,
+                                              const CLists& W, int sg, int tile, unsigned char* smem_raw) {
   using C = typename Mo::C;
   using S = typename Mo::S;
-  extern __shared__ unsigned char smem_raw[];
   C* psl = reinterpret_cast<C*>(smem_raw);                                            // kSlots x PYZ
   uint32_t* rm = reinterpret_cast<uint32_t*>(smem_raw + kCompactSlots * T.PYZ * sizeof(C));  // kSlots x RYZ
   int* ycrd = reinterpret_cast<int*>(rm + kCompactSlots * T.RYZ);                    // TYi + 4 rows
@@ -862,15 +861,6 @@ __global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typena
   uint32_t* qtab = reinterpret_cast<uint32_t*>(zcrd + (T.TZi + 4));          // RYZ: rank word << 5 | bit
   int* qpsi = reinterpret_cast<int*>(qtab + T.RYZ);                          // RYZ: psi index or -1
 
-  int b = blockIdx.x;
-  if (T.xcd_remap) {
-    const int per = gridDim.x >> 3;
-    b = (b & 7) * per + (b >> 3);
-  }
-  if (b >= T.nblocks) return;  // whole padding block leaves before any barrier
-  const int ntile = T.tiles_y * T.tiles_z;
-  const int sg = b / ntile;
-  const int tile = b - sg * ntile;
   const int tyi = tile / T.tiles_z;
   const int tzi = tile - tyi * T.tiles_z;
   const int y0 = tyi * T.TYi, z0 = tzi * T.TZi;
@@ -1113,6 +1103,21 @@ __global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typena
       collide_node(x, p, f, rho, ycrd[r + 1], zcrd[p - r * T.PZ + 1], own);
     }
   }
+}
+
+template <class Mo, bool kFaces, int kThreads>
+__global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typename Mo::C> ph, CTiling T,
+                                                      CLists W) {
+  extern __shared__ unsigned char smem_raw[];
+  int b = blockIdx.x;
+  if (T.xcd_remap) {
+    const int per = gridDim.x >> 3;
+    b = (b & 7) * per + (b >> 3);
+  }
+  if (b >= T.nblocks) return;  // whole padding block leaves before any barrier
+  const int ntile = T.tiles_y * T.tiles_z;
+  const int sg = b / ntile;
+  compact_march<Mo, kFaces, kThreads>(L, ph, T, W, sg, b - sg * ntile, smem_raw);
 }
 
 // Store population i of the fluid node of rank r in plane x of a compact
@@ -2269,10 +2274,39 @@ __global__ __launch_bounds__(kBlock) void k_ensemble(const EnsItem* __restrict__
   }
 }
 
+template <class Mo>
+struct EnsSampleC {
+  Lattice<Mo> L[2];  // [parity], as EnsSample
+  Phys<typename Mo::C> ph;
+  CTiling T;
+  CLists W;
+};
+
+// Compact form of the persistent launch: each item is a compact march of
+// one sample's tile and x segment.
+template <class Mo, int kBlock>
+__global__ __launch_bounds__(kBlock) void k_ensemble_c(const EnsItem* __restrict__ items, int nitems,
+                                                       const EnsSampleC<Mo>* __restrict__ samples, int parity,
+                                                       int macro) {
+  extern __shared__ unsigned char smem_raw[];
+  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    const EnsItem it = items[w];
+    const int s = __builtin_amdgcn_readfirstlane(it.sample);
+    const EnsSampleC<Mo>& sd = samples[s];
+    Phys<typename Mo::C> ph = sd.ph;
+    ph.write_macro = macro;
+    const int tile = __builtin_amdgcn_readfirstlane(it.tyi * sd.T.tiles_z + it.tzi);
+    compact_march<Mo, false, kBlock>(sd.L[parity], ph, sd.T, sd.W, __builtin_amdgcn_readfirstlane(it.seg), tile,
+                                     smem_raw);
+    __syncthreads();  // the next item reuses the LDS regions
+  }
+}
+
 }  // namespace mplbm
 
 struct mplbm_ensemble {
   std::vector<mplbm_ctx*> ctx;
+  bool compact = true;  // fluid-node layout and compact marching
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int block = 512;
@@ -2326,6 +2360,20 @@ static mplbm_status ens_plan(mplbm_ensemble* e) {
   const size_t csize = (e->storage == MPLBM_FP64) ? 8 : 4;
   for (size_t s = 0; s < e->ctx.size(); ++s) {
     mplbm_ctx* c = e->ctx[s];
+    if (e->compact) {
+      // interior tile of at most `block` nodes, 32 wide in z where the axis
+      // allows; narrower in y until the LDS regions fit
+      const int tz = (int)std::min<int64_t>(32, c->cfg.lz);
+      int ty = (int)std::min<int64_t>(c->cfg.ly, std::max(1, std::min(e->block / tz, 16)));
+      const int seg = (int)std::min<int64_t>(32, c->cfg.lx);
+      mplbm_status st = MPLBM_ERR_ARG;
+      for (; ty >= 1; --ty)
+        if ((st = mplbm_set_tiling(c, ty, tz, seg)) == MPLBM_OK) break;
+      if (st != MPLBM_OK) return ens_fail(e, st, "sample %zu: %s", s, c->err.c_str());
+      e->lds_bytes = std::max(e->lds_bytes, csize == 8 ? compact_lds_bytes<double>(c->ctiling)
+                                                       : compact_lds_bytes<float>(c->ctiling));
+      continue;
+    }
     int ty, tz, seg;
     ens_fit_tile(c->cfg, e->block, &ty, &tz, &seg);
     mplbm_status st = mplbm_set_tiling(c, ty, tz, seg);
@@ -2337,14 +2385,17 @@ static mplbm_status ens_plan(mplbm_ensemble* e) {
   }
   // items ordered by segment, then sample, then tile: consecutive workgroups
   // march neighbouring tiles of one sample, so ring re-reads meet in L2
+  auto nseg_of = [&](mplbm_ctx* c) { return e->compact ? c->ctiling.nseg : c->tiling.nseg; };
   int max_seg = 0;
-  for (mplbm_ctx* c : e->ctx) max_seg = std::max(max_seg, c->tiling.nseg);
+  for (mplbm_ctx* c : e->ctx) max_seg = std::max(max_seg, nseg_of(c));
   for (int g = 0; g < max_seg; ++g)
     for (size_t s = 0; s < e->ctx.size(); ++s) {
-      const Tiling& T = e->ctx[s]->tiling;
-      if (g >= T.nseg) continue;
-      for (int a = 0; a < T.tiles_y; ++a)
-        for (int b = 0; b < T.tiles_z; ++b) e->items.push_back(EnsItem{(int)s, g, a, b});
+      mplbm_ctx* c = e->ctx[s];
+      if (g >= nseg_of(c)) continue;
+      const int ty_n = e->compact ? c->ctiling.tiles_y : c->tiling.tiles_y;
+      const int tz_n = e->compact ? c->ctiling.tiles_z : c->tiling.tiles_z;
+      for (int a = 0; a < ty_n; ++a)
+        for (int b = 0; b < tz_n; ++b) e->items.push_back(EnsItem{(int)s, g, a, b});
     }
   if (e->d_items) (void)hipFree(e->d_items);
   e->d_items = nullptr;
@@ -2372,18 +2423,39 @@ static mplbm_status ens_dispatch(mplbm_ensemble* e, F&& f) {
 
 // Upload the per-sample descriptor table for the lattices current now;
 // inject_now follows `step`.
+template <class Mo, class Tab>
+static void ens_fill(mplbm_ensemble* e, int64_t step, Tab& tab) {
+  for (size_t s = 0; s < e->ctx.size(); ++s) {
+    mplbm_ctx* c = e->ctx[s];
+    tab[s].L[0] = make_lattice<Mo>(c, c->cur, 1 - c->cur);
+    tab[s].L[1] = make_lattice<Mo>(c, 1 - c->cur, c->cur);
+    tab[s].ph = make_phys<typename Mo::C>(c, step, 1, 0, false);
+  }
+}
+
 static mplbm_status ens_upload(mplbm_ensemble* e, int64_t step) {
   return ens_dispatch(e, [&](auto mo) -> mplbm_status {
     using Mo = decltype(mo);
-    std::vector<EnsSample<Mo>> tab(e->ctx.size());
-    for (size_t s = 0; s < e->ctx.size(); ++s) {
-      mplbm_ctx* c = e->ctx[s];
-      tab[s].L[0] = make_lattice<Mo>(c, c->cur, 1 - c->cur);
-      tab[s].L[1] = make_lattice<Mo>(c, 1 - c->cur, c->cur);
-      tab[s].ph = make_phys<typename Mo::C>(c, step, 1, 0, false);
-      tab[s].T = c->tiling;
+    std::vector<EnsSample<Mo>> tab;
+    std::vector<EnsSampleC<Mo>> tabc;
+    const void* src = nullptr;
+    size_t bytes = 0;
+    if (e->compact) {
+      tabc.resize(e->ctx.size());
+      ens_fill<Mo>(e, step, tabc);
+      for (size_t s = 0; s < e->ctx.size(); ++s) {
+        tabc[s].T = e->ctx[s]->ctiling;
+        tabc[s].W = CLists{e->ctx[s]->cpos, e->ctx[s]->ctab};
+      }
+      src = tabc.data();
+      bytes = tabc.size() * sizeof(EnsSampleC<Mo>);
+    } else {
+      tab.resize(e->ctx.size());
+      ens_fill<Mo>(e, step, tab);
+      for (size_t s = 0; s < e->ctx.size(); ++s) tab[s].T = e->ctx[s]->tiling;
+      src = tab.data();
+      bytes = tab.size() * sizeof(EnsSample<Mo>);
     }
-    const size_t bytes = tab.size() * sizeof(EnsSample<Mo>);
     if (e->sample_bytes != bytes) {
       if (e->d_samples) (void)hipFree(e->d_samples);
       e->d_samples = nullptr;
@@ -2394,9 +2466,9 @@ static mplbm_status ens_upload(mplbm_ensemble* e, int64_t step) {
       e->graph_exec = nullptr;
       e->graph = nullptr;
     }
-    // stream-ordered after earlier steps; synchronous so `tab` can go away
+    // stream-ordered after earlier steps; synchronous so the table can go away
     ENS_HIP(e, hipStreamSynchronize(e->stream));
-    ENS_HIP(e, hipMemcpy(e->d_samples, tab.data(), bytes, hipMemcpyHostToDevice));
+    ENS_HIP(e, hipMemcpy(e->d_samples, src, bytes, hipMemcpyHostToDevice));
     return MPLBM_OK;
   });
 }
@@ -2410,6 +2482,20 @@ static mplbm_status ens_launch(mplbm_ensemble* e, int parity, int macro) {
     const int nitems = (int)e->items.size();
     const int per_cu = e->block >= 1024 ? 2 : (e->block >= 512 ? 3 : 6);
     const int grid = std::max(1, std::min(nitems, ncu * per_cu));
+    if (e->compact) {
+      const auto* sc = static_cast<const EnsSampleC<Mo>*>(e->d_samples);
+      if (e->block == 512)
+        hipLaunchKernelGGL((k_ensemble_c<Mo, 512>), dim3(grid), dim3(512), e->lds_bytes, e->stream, e->d_items,
+                           nitems, sc, parity, macro);
+      else if (e->block == 256)
+        hipLaunchKernelGGL((k_ensemble_c<Mo, 256>), dim3(grid), dim3(256), e->lds_bytes, e->stream, e->d_items,
+                           nitems, sc, parity, macro);
+      else if constexpr (Mo::kStorage != MPLBM_FP64)
+        hipLaunchKernelGGL((k_ensemble_c<Mo, 1024>), dim3(grid), dim3(1024), e->lds_bytes, e->stream, e->d_items,
+                           nitems, sc, parity, macro);
+      ENS_HIP(e, hipGetLastError());
+      return MPLBM_OK;
+    }
     const auto* samp = static_cast<const EnsSample<Mo>*>(e->d_samples);
     if (e->block == 1024)
       hipLaunchKernelGGL((k_ensemble<Mo, 1024>), dim3(grid), dim3(1024), e->lds_bytes, e->stream, e->d_items,
@@ -2434,6 +2520,11 @@ mplbm_status mplbm_ensemble_create(const mplbm_config* cfgs, int32_t n, int32_t 
   if (block_threads == 0) block_threads = 512;
   if (block_threads != 256 && block_threads != 512 && block_threads != 1024)
     return ens_fail(nullptr, MPLBM_ERR_ARG, "block_threads must be 256, 512 or 1024");
+  if (cfgs[0].kernel == MPLBM_KERNEL_TWOPASS)
+    return ens_fail(nullptr, MPLBM_ERR_UNSUPPORTED, "the ensemble runs the compact or the fused kernel");
+  const bool compact = cfgs[0].kernel != MPLBM_KERNEL_FUSED;
+  if (compact && cfgs[0].storage == MPLBM_FP64 && block_threads > 512)
+    return ens_fail(nullptr, MPLBM_ERR_ARG, "FP64 compact ensemble needs block_threads <= 512");
   for (int32_t s = 0; s < n; ++s) {
     if (cfgs[s].storage != cfgs[0].storage || cfgs[s].math != cfgs[0].math)
       return ens_fail(nullptr, MPLBM_ERR_ARG, "samples must share storage and arithmetic");
@@ -2442,12 +2533,13 @@ mplbm_status mplbm_ensemble_create(const mplbm_config* cfgs, int32_t n, int32_t 
         return ens_fail(nullptr, MPLBM_ERR_UNSUPPORTED, "ensemble samples are periodic");
   }
   mplbm_ensemble* e = new mplbm_ensemble();
+  e->compact = compact;
   e->block = block_threads;
   e->storage = cfgs[0].storage;
   e->math = cfgs[0].math;
   for (int32_t s = 0; s < n; ++s) {
     mplbm_config c = cfgs[s];
-    c.kernel = MPLBM_KERNEL_FUSED;
+    c.kernel = compact ? MPLBM_KERNEL_COMPACT : MPLBM_KERNEL_FUSED;
     mplbm_ctx* ctx = nullptr;
     mplbm_status st = mplbm_create(&c, &ctx);
     if (st != MPLBM_OK) {

@@ -8,7 +8,7 @@ import ctypes
 import numpy as np
 
 from . import _abi
-from . import MplbmError, STORAGE_NAMES, default_config
+from . import KERNEL_NAMES, MplbmError, STORAGE_NAMES, default_config
 
 VP = ctypes.c_void_p
 SIGNATURES = {
@@ -50,9 +50,10 @@ def _p(a):
 
 class Ensemble:
     """samples: list of dicts with keys lx, ly, lz and any mplbm_config
-    physics field (G, tau, rhos, ...).  storage/math are shared."""
+    physics field (G, tau, rhos, ...).  storage/math are shared; kernel is
+    "compact" (fluid nodes only, the default) or "fused" (full lattice)."""
 
-    def __init__(self, samples, storage="fp32", math="reference", block_threads=512):
+    def __init__(self, samples, storage="fp32", math="reference", block_threads=512, kernel="compact"):
         self.lib = _lib()
         n = len(samples)
         arr = (_abi.Config * n)()
@@ -61,6 +62,7 @@ class Ensemble:
             c = default_config()
             c.storage = STORAGE_NAMES[storage]
             c.math = {"reference": 0, "fast": 1}[math]
+            c.kernel = KERNEL_NAMES[kernel]
             for key, v in s.items():
                 setattr(c, key, v)
             arr[k] = c

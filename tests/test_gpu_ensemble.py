@@ -25,12 +25,13 @@ def _inputs(k, s):
     return shape, fluid, rho
 
 
+@pytest.mark.parametrize("kernel", ["compact", "fused"])
 @pytest.mark.parametrize("block", [256, 512, 1024])
 @pytest.mark.parametrize("use_graph", [True, False])
-def test_ensemble_samples_match_oracle(block, use_graph):
-    print(f"ensemble block={block} graph={use_graph}", flush=True)
+def test_ensemble_samples_match_oracle(block, use_graph, kernel):
+    print(f"ensemble {kernel} block={block} graph={use_graph}", flush=True)
     steps = 9
-    with Ensemble(SAMPLES, block_threads=block) as ens:
+    with Ensemble(SAMPLES, block_threads=block, kernel=kernel) as ens:
         data = []
         for k, s in enumerate(SAMPLES):
             shape, fluid, rho = _inputs(k, s)
