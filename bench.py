@@ -61,6 +61,10 @@ def parse():
                     help="batched ensemble: 64 packings n^3, n~U{48..160}, one persistent launch per step")
     ap.add_argument("--ensemble-samples", type=int, default=64)
     ap.add_argument("--ensemble-steps", type=int, default=5000, help="steps before the saturation read-out")
+    ap.add_argument("--drainage", action="store_true",
+                    help="768x512x512 packing between a liquid inlet and a vapour outlet: FP32 vs FP16 throughput, "
+                         "and FP64/FP32/FP16 saturation on a 1/8 sub-volume")
+    ap.add_argument("--drainage-steps", type=int, default=20000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain device-to-device copy kernel of 2 GiB (rocprofv3 --pmc calibration)")
     return ap.parse_args()
@@ -312,6 +316,86 @@ def ensemble_bench(a):
     print(json.dumps(out), flush=True)
 
 
+def drainage_fields(shape, seed, porosity):
+    """Packing (porosity, r ~ U[8, 20]) and the initial density: liquid
+    rho = 1.9 in the first lx/16 planes (next to the inlet) and in seeded
+    liquid spheres up to 3/4 of the length, vapour 0.15 elsewhere (the last
+    quarter, next to the outlet, starts as vapour)."""
+    from multiphase_lbm_amd import harness as H
+    fluid, _ = H.generate_packing(shape, seed, 10 ** 7, 8.0, 20.0, porosity)
+    blobs, _ = H.generate_packing(shape, seed + 1000, 10 ** 7, 6.0, 14.0, 0.85)
+    rho = np.full(shape, 0.15, np.float64)
+    rho[blobs == 0] = 1.9
+    rho[(3 * shape[0]) // 4:] = 0.15
+    rho[: max(1, shape[0] // 16)] = 1.9
+    return fluid, rho
+
+
+DRAIN = dict(tau=1.0, G=-5.0, rhos=0.7, rhol_spinodal=1.285931, rhog_spinodal=0.323507,
+             face_type=(1, 1, 0, 0, 0, 0), face_rho=(2.0, 0.15, 0, 0, 0, 0))
+
+
+def drainage_bench(a):
+    """Config of the large drainage-style run: 768x512x512, porosity 0.4,
+    r ~ U[8, 20], pressure inlet rho = 2.0 at x = 0 and outlet rho = 0.15 at
+    x = lx-1, y/z periodic.  (1) FP32 and FP16 storage throughput on the full
+    domain; (2) FP64, FP32, FP16 from the same start on a 1/8 sub-volume
+    (384x256x256) for `--drainage-steps` steps: saturation and density error
+    against FP64.  One JSON line."""
+    import multiphase_lbm_amd as mp
+    thr = 0.5 * (1.932442 + 0.156413)
+    out = {"workload": "768x512x512 packing, porosity 0.4, r~U[8,20], inlet rho 2.0 / outlet rho 0.15 on x, "
+                       "liquid 1.9 / vapour 0.15 start"}
+    shape = (768, 512, 512)
+    t = time.time()
+    fluid, rho = drainage_fields(shape, 5, 0.4)
+    print(f"# drainage domain built in {time.time() - t:.1f}s, porosity {fluid.mean():.4f}", file=sys.stderr,
+          flush=True)
+    for storage in a.storage.split(","):
+        with mp.Context(*shape, storage=storage, kernel=a.kernel, **DRAIN) as ctx:
+            ctx.set_geometry(fluid)
+            ctx.init_fields(rho.astype(np.float32))
+            nf = ctx.info()["fluid_nodes"]
+            ctx.step(0, a.warmup)
+            ms = ctx.step_timed(a.warmup, a.steps) / a.steps
+            st = ctx.reduce()
+        bpu = 2 * Q * mp.STORAGE_BYTES[mp.STORAGE_NAMES[storage.replace("fast", "")]]
+        out[storage] = {"ms_per_step": round(ms, 4), "mlups": round(nf / ms / 1e3, 1),
+                        "roofline_frac": round(nf * bpu / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "fluid_nodes": nf, "nonfinite": st["nonfinite"]}
+        print(f"# drainage {storage}: {out[storage]}", file=sys.stderr, flush=True)
+    del fluid, rho
+    sub = (384, 256, 256)
+    sfluid, srho = drainage_fields(sub, 6, 0.4)
+    sfl = sfluid.astype(bool)
+    res = {}
+    for storage in ("fp64", "fp32", "fp16s"):
+        t = time.time()
+        with mp.Context(*sub, storage=storage, kernel=a.kernel, **DRAIN) as ctx:
+            ctx.set_geometry(sfluid)
+            ctx.init_fields(srho if storage == "fp64" else srho.astype(np.float32))
+            done = 0
+            while done < a.drainage_steps:
+                k = min(2000, a.drainage_steps - done)
+                ctx.step(done, k, write_macro=done + k >= a.drainage_steps)
+                ctx.sync()
+                done += k
+                print(f"# sub-volume {storage}: {done} steps", file=sys.stderr, flush=True)
+            r, _, _ = ctx.read_fields(np.float64)
+            st = ctx.reduce()
+        res[storage] = (r, st, time.time() - t)
+    ref = res["fp64"][0]
+    sv = {"shape": list(sub), "steps": a.drainage_steps, "fp64": {"saturation": float((ref[sfl] > thr).mean())}}
+    for storage in ("fp32", "fp16s"):
+        r, st, dt = res[storage]
+        sv[storage] = {"saturation": float((r[sfl] > thr).mean()),
+                       "saturation_abs_err": abs(float((r[sfl] > thr).mean()) - sv["fp64"]["saturation"]),
+                       "density_rel_l2": float(np.linalg.norm(r[sfl] - ref[sfl]) / np.linalg.norm(ref[sfl])),
+                       "nonfinite": st["nonfinite"], "seconds": round(dt, 1)}
+    out["sub_volume"] = sv
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -323,6 +407,9 @@ def main():
         sys.exit(2)
     if a.ensemble:
         ensemble_bench(a)
+        return
+    if a.drainage:
+        drainage_bench(a)
         return
     shape, fluid, rho, nsph = workload(a)
     if a.precision:

@@ -824,7 +824,8 @@ struct CTiling {
   int seg, nseg;         // planes per workgroup, segments along x
   int nblocks;
   int xcd_remap;
-  int threads;           // workgroup size (256 or 512)
+  int threads;           // workgroup size (256, 512 or 1024)
+  int seg0, seg_stride;  // a launch covers segments seg0 + k*seg_stride, k < nblocks / tiles
 };
 
 struct CLists {
@@ -1116,8 +1117,8 @@ __global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typena
   }
   if (b >= T.nblocks) return;  // whole padding block leaves before any barrier
   const int ntile = T.tiles_y * T.tiles_z;
-  const int sg = b / ntile;
-  compact_march<Mo, kFaces, kThreads>(L, ph, T, W, sg, b - sg * ntile, smem_raw);
+  const int k = b / ntile;
+  compact_march<Mo, kFaces, kThreads>(L, ph, T, W, T.seg0 + k * T.seg_stride, b - k * ntile, smem_raw);
 }
 
 // Store population i of the fluid node of rank r in plane x of a compact
@@ -1511,6 +1512,8 @@ static mplbm_status choose_ctiling(mplbm_ctx* ctx) {
   if (nb > (1LL << 30)) return fail(ctx, MPLBM_ERR_ARG, "too many blocks");
   T.nblocks = (int)nb;
   T.xcd_remap = 1;
+  T.seg0 = 0;
+  T.seg_stride = 1;
   if (c.block_threads != 0 && c.block_threads != 256 && c.block_threads != 512 && c.block_threads != 1024)
     return fail(ctx, MPLBM_ERR_ARG, "block_threads must be 0, 256, 512 or 1024");
   {
@@ -1648,26 +1651,45 @@ static mplbm_status launch_step(mplbm_ctx* ctx, int64_t step, int32_t reverse, u
   const Lattice<Mo> L = make_lattice<Mo>(ctx, ctx->cur, 1 - ctx->cur);
   if (ctx->kernel == MPLBM_KERNEL_COMPACT) {
     const CTiling& T = ctx->ctiling;
-    const int grid = T.xcd_remap ? ((T.nblocks + 7) / 8) * 8 : T.nblocks;
-    const size_t shm = compact_lds_bytes<C>(T);
     const CLists WL{ctx->cpos, ctx->ctab};
+    const size_t shm = compact_lds_bytes<C>(T);
     // the workgroup size is at least the tile's interior count (one kept
     // interior node per thread); 1024 only for non-FP64 storage
-    if (T.threads == 256) {
-      if (ph.any_face)
-        hipLaunchKernelGGL((k_compact<Mo, true, 256>), dim3(grid), dim3(256), shm, ctx->stream, L, ph, T, WL);
-      else
-        hipLaunchKernelGGL((k_compact<Mo, false, 256>), dim3(grid), dim3(256), shm, ctx->stream, L, ph, T, WL);
-    } else if (T.threads == 512) {
-      if (ph.any_face)
-        hipLaunchKernelGGL((k_compact<Mo, true, 512>), dim3(grid), dim3(512), shm, ctx->stream, L, ph, T, WL);
-      else
-        hipLaunchKernelGGL((k_compact<Mo, false, 512>), dim3(grid), dim3(512), shm, ctx->stream, L, ph, T, WL);
-    } else if constexpr (Mo::kStorage != MPLBM_FP64) {
-      if (ph.any_face)
-        hipLaunchKernelGGL((k_compact<Mo, true, 1024>), dim3(grid), dim3(1024), shm, ctx->stream, L, ph, T, WL);
-      else
-        hipLaunchKernelGGL((k_compact<Mo, false, 1024>), dim3(grid), dim3(1024), shm, ctx->stream, L, ph, T, WL);
+    auto launch = [&](const CTiling& R, bool faces) {
+      const int grid = R.xcd_remap ? ((R.nblocks + 7) / 8) * 8 : R.nblocks;
+      if (R.threads == 256) {
+        if (faces)
+          hipLaunchKernelGGL((k_compact<Mo, true, 256>), dim3(grid), dim3(256), shm, ctx->stream, L, ph, R, WL);
+        else
+          hipLaunchKernelGGL((k_compact<Mo, false, 256>), dim3(grid), dim3(256), shm, ctx->stream, L, ph, R, WL);
+      } else if (R.threads == 512) {
+        if (faces)
+          hipLaunchKernelGGL((k_compact<Mo, true, 512>), dim3(grid), dim3(512), shm, ctx->stream, L, ph, R, WL);
+        else
+          hipLaunchKernelGGL((k_compact<Mo, false, 512>), dim3(grid), dim3(512), shm, ctx->stream, L, ph, R, WL);
+      } else if constexpr (Mo::kStorage != MPLBM_FP64) {
+        if (faces)
+          hipLaunchKernelGGL((k_compact<Mo, true, 1024>), dim3(grid), dim3(1024), shm, ctx->stream, L, ph, R, WL);
+        else
+          hipLaunchKernelGGL((k_compact<Mo, false, 1024>), dim3(grid), dim3(1024), shm, ctx->stream, L, ph, R, WL);
+      }
+    };
+    const bool x_faces_only = ph.any_face && !(ph.face[2] | ph.face[3] | ph.face[4] | ph.face[5]);
+    if (x_faces_only && T.nseg >= 3 && T.seg >= 2) {
+      // face rules and the skipped force only touch planes 0 and lx-1: the
+      // first and last segments run the face variant, the others the plain
+      // one (fewer registers), in two launches
+      const int ntile = T.tiles_y * T.tiles_z;
+      CTiling mid = T, ends = T;
+      mid.seg0 = 1;
+      mid.nblocks = ntile * (T.nseg - 2);
+      ends.seg0 = 0;
+      ends.seg_stride = T.nseg - 1;
+      ends.nblocks = ntile * 2;
+      launch(mid, false);
+      launch(ends, true);
+    } else {
+      launch(T, ph.any_face != 0);
     }
   } else if (ctx->kernel == MPLBM_KERNEL_FUSED) {
     const Tiling& T = ctx->tiling;

@@ -174,13 +174,19 @@ def test_carnahan_starling(kernel):
     assert np.array_equal(g_f[fluid], st.fA[fluid].astype(np.float64))
 
 
-@pytest.mark.parametrize("kernel", ["fused", "twopass", "compact"])
-def test_pressure_faces_and_body_force(kernel):
+@pytest.mark.parametrize("kernel,seg", [("fused", 0), ("twopass", 0), ("compact", 0), ("compact", 16),
+                                        ("compact", 7)])
+def test_pressure_faces_and_body_force(kernel, seg):
+    """x faces only: the compact kernel runs its face variant on the first
+    and last segments and the plain variant elsewhere (seg 0: 4-plane
+    segments; 16: one segment; 7: a short last segment)."""
     shape = (16, 12, 10)
     fluid, rho = _case(shape, 2, 2.0, 3.0, seed=9, rho0=1.0, amp=0.05)
-    print(f"case pressure faces {kernel}", flush=True)
+    print(f"case pressure faces {kernel} seg {seg}", flush=True)
     params = dict(SC, G=-3.0, face_type=(1, 1, 0, 0, 0, 0), face_rho=(1.05, 0.95, 0, 0, 0, 0),
                   body_force=(1e-5, 0, 2e-6))
+    if kernel == "compact":
+        params["seg_x"] = seg
     st, (g_rho, g_p, g_u, g_f, _) = _run_pair(shape, fluid, rho, 15, "fp32", kernel, **params)
     assert np.isfinite(st.fA[fluid]).all()
     assert np.array_equal(g_f[fluid], st.fA[fluid].astype(np.float64))
@@ -238,3 +244,30 @@ def test_fp32_fast_math_error_vs_oracle():
     rel_u = _abserr(g_u[fluid], st.u[fluid]) / umax
     print(f"fp32 fast vs oracle after 100 steps: max rel density {rel_rho:.3e}, max |du|/max|u| {rel_u:.3e}")
     assert rel_rho <= 1e-3
+
+
+@pytest.mark.parametrize("kernel", ["compact", "fused"])
+def test_drainage_faces_two_phase(kernel):
+    """The drainage set-up at small size: pressure inlet 2.0 / outlet 0.15 on
+    x, G = -5, liquid blobs in vapour (bench.py --drainage)."""
+    from bench import DRAIN, drainage_fields
+    shape = (40, 24, 20)
+    fluid, rho = drainage_fields(shape, 5, 0.4)
+    fluid = fluid.astype(bool)
+    print(f"case drainage {kernel}", flush=True)
+    st, (g_rho, g_p, g_u, g_f, stats) = _run_pair(shape, fluid, rho.astype(np.float32), 30, "fp32", kernel, **DRAIN)
+    assert np.isfinite(st.fA[fluid]).all()
+    assert np.array_equal(g_f[fluid], st.fA[fluid].astype(np.float64))
+    assert np.array_equal(g_rho[fluid], st.rho[fluid].astype(np.float64))
+
+
+@pytest.mark.parametrize("kernel", ["fused", "compact"])
+def test_pressure_faces_on_y(kernel):
+    """Faces on y (every segment runs the face variant)."""
+    shape = (10, 14, 12)
+    fluid, rho = _case(shape, 2, 2.0, 3.0, seed=19, rho0=1.0, amp=0.05)
+    print(f"case pressure faces y {kernel}", flush=True)
+    params = dict(SC, G=-3.0, face_type=(0, 0, 1, 1, 0, 0), face_rho=(0, 0, 1.04, 0.96, 0, 0))
+    st, (g_rho, g_p, g_u, g_f, _) = _run_pair(shape, fluid, rho, 12, "fp32", kernel, **params)
+    assert np.isfinite(st.fA[fluid]).all()
+    assert np.array_equal(g_f[fluid], st.fA[fluid].astype(np.float64))
