@@ -109,6 +109,8 @@ struct Phys {
   C inject_ylim;    // inject_position * ly
   C g[3];           // body acceleration
   float rho_ref, scale, inv_scale;  // deviation storage
+  float dec_mul[3], dec_add[3];      // per weight class: f = g * dec_mul + dec_add
+  float enc_mul[3], enc_add;         // g = f * enc_mul + enc_add
   int inject_now;   // step % rho_inject_period == 0
   int inject_type;
   int has_body;
@@ -129,7 +131,8 @@ __device__ __forceinline__ typename Mo::C decode(typename Mo::S s, int i, const 
     float g;
     if constexpr (Mo::kStorage == MPLBM_FP16S) g = __half2float(s);
     else g = __bfloat162float(s);
-    return weight<float>(i) * __builtin_fmaf(g, ph.inv_scale, ph.rho_ref);
+    // w_i (g / scale + rho_ref) folded into one multiply-add per class
+    return __builtin_fmaf(g, ph.dec_mul[wclass(i)], ph.dec_add[wclass(i)]);
   }
 }
 
@@ -138,7 +141,8 @@ __device__ __forceinline__ typename Mo::S encode(typename Mo::C v, int i, const 
   if constexpr (!Mo::kDev) {
     return v;
   } else {
-    const float g = __builtin_fmaf(v, inv_weight_f(i), -ph.rho_ref) * ph.scale;
+    // (f / w_i - rho_ref) * scale folded into one multiply-add per class
+    const float g = __builtin_fmaf(v, ph.enc_mul[wclass(i)], ph.enc_add);
     if constexpr (Mo::kStorage == MPLBM_FP16S) return __float2half_rn(g);
     else return __float2bfloat16(g);
   }
@@ -1399,6 +1403,13 @@ static Phys<C> make_phys(const mplbm_ctx* ctx, int64_t step, int32_t reverse, ui
   ph.rho_ref = ctx->rho_ref;
   ph.scale = ctx->scale;
   ph.inv_scale = 1.0f / ctx->scale;
+  for (int k = 0; k < 3; ++k) {
+    const int i = k == 0 ? 0 : (k == 1 ? 1 : 4);  // a direction of each weight class
+    ph.dec_mul[k] = weight<float>(i) * ph.inv_scale;
+    ph.dec_add[k] = weight<float>(i) * ph.rho_ref;
+    ph.enc_mul[k] = inv_weight_f(i) * ph.scale;
+  }
+  ph.enc_add = -ph.rho_ref * ph.scale;
   ph.write_macro = write_macro ? 1 : 0;
   return ph;
 }
