@@ -130,6 +130,20 @@ typedef struct mplbm_stats64 {
   double nonfinite;            /* populations that are NaN or Inf */
 } mplbm_stats64;
 
+/* Statistics of MultiphaseLBM::calc_cluster (src/multiphase_lbm.cu:225-562)
+ * with the extrema of write_stats (:1057-1083), from the last macro write
+ * (mplbm_calc_cluster).  Same fields as the host harness's
+ * mplbm_cluster_stats. */
+typedef struct mplbm_clusters {
+  double min_rho, max_rho, min_p, max_p;
+  double avg_liq_press, avg_vap_press, saturation, avg_liq_rho, avg_vap_rho;
+  int64_t liq_cluster_count, gas_cluster_count;
+  double cluster_order, wet_coord_num;
+  double ratio_liquid_solid_boundary, ratio_liquid_vapor_boundary;
+  double cluster_radius;
+  int64_t fluid_count, liquid_count, vapor_count;
+} mplbm_clusters;
+
 /* Read-only facts about a context (mplbm_info). */
 typedef struct mplbm_info {
   int64_t fluid_nodes;
@@ -212,6 +226,28 @@ mplbm_status mplbm_write_populations_f64(mplbm_ctx* ctx, const double* f);
 /* FP64 reductions of the newest lattice and of the last macro write
  * (synchronises). */
 mplbm_status mplbm_reduce(mplbm_ctx* ctx, mplbm_stats64* out);
+
+/* Solid numbering for the cluster statistics: host [x][y][z] array as
+ * place_sphere writes it (src/multiphase_lbm.cu:893-927: 0 at fluid cells,
+ * the 1-based sphere id at solid cells, the last sphere written wins), and
+ * the number of solids (the wet coordination number's divisor).  Copied to
+ * the device once.  NULL clears it (cluster order and wet coordination
+ * number are then 0). */
+mplbm_status mplbm_set_solid_numbers(mplbm_ctx* ctx, const int64_t* solid_number, int64_t solid_count);
+
+/* calc_cluster on the device (src/multiphase_lbm.cu:225-562; the reference
+ * runs it on the host at every statPeriod after reading rho and p through
+ * unified memory, :243-262): phase counts, FP64 phase sums, extrema,
+ * liquid-solid and liquid-vapour boundary ratios, liquid and vapour cluster
+ * labelling (6-neighbour, periodic, clusters of fewer than 7 nodes dropped,
+ * numbered in the order of their first cell with z fastest), cluster order,
+ * wet coordination number and the single-cluster radius, from the fields of
+ * the last macro write (mplbm_step with MPLBM_STEP_WRITE_MACRO).
+ * liq_label / gas_label (host [x][y][z], may be NULL) receive the cluster
+ * numbers of the VTK LiqClusters / GasClusters blocks.  Synchronises.
+ * Needs fewer than 2^31 cells. */
+mplbm_status mplbm_calc_cluster(mplbm_ctx* ctx, double rhol_spinodal, double rhog_spinodal, mplbm_clusters* out,
+                                 int32_t* liq_label, int32_t* gas_label);
 
 /* Changes the FUSED or COMPACT kernel's tile (see mplbm_config.tile_y) and
  * planes per block; 0 keeps the tuned default for that field.  Used by the

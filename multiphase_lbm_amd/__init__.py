@@ -154,6 +154,28 @@ class Context:
         self._check(self.lib.mplbm_reduce(self.h, ctypes.byref(s)))
         return s.as_dict()
 
+    def set_solid_numbers(self, solid_number, solid_count):
+        """Solid numbering ([x][y][z], 0 at fluid cells) for calc_cluster."""
+        sn = None
+        if solid_number is not None:
+            sn = np.ascontiguousarray(solid_number, np.int64)
+            if sn.shape != self.shape:
+                raise ValueError(f"solid_number shape {sn.shape} != {self.shape}")
+        self._check(self.lib.mplbm_set_solid_numbers(self.h, None if sn is None else _ptr(sn), int(solid_count)))
+
+    def calc_cluster(self, rhol, rhog, want_labels=False):
+        """calc_cluster statistics of the last macro write on the device;
+        with want_labels also the LiqClusters / GasClusters arrays."""
+        s = _abi.Clusters()
+        liq = np.zeros(self.shape, np.int32) if want_labels else None
+        gas = np.zeros(self.shape, np.int32) if want_labels else None
+        self._check(self.lib.mplbm_calc_cluster(self.h, float(rhol), float(rhog), ctypes.byref(s),
+                                                None if liq is None else _ptr(liq),
+                                                None if gas is None else _ptr(gas)))
+        if want_labels:
+            return s.as_dict(), liq, gas
+        return s.as_dict()
+
     def set_tiling(self, tile_y=0, tile_z=0, seg_x=0):
         self._check(self.lib.mplbm_set_tiling(self.h, int(tile_y), int(tile_z), int(seg_x)))
 

@@ -62,6 +62,19 @@ class Info(ctypes.Structure):
     ]
 
 
+class Clusters(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "min_rho", "max_rho", "min_p", "max_p", "avg_liq_press", "avg_vap_press", "saturation",
+        "avg_liq_rho", "avg_vap_rho")] + [
+        ("liq_cluster_count", ctypes.c_int64), ("gas_cluster_count", ctypes.c_int64)] + [
+        (n, ctypes.c_double) for n in ("cluster_order", "wet_coord_num", "ratio_liquid_solid_boundary",
+                                       "ratio_liquid_vapor_boundary", "cluster_radius")] + [
+        ("fluid_count", ctypes.c_int64), ("liquid_count", ctypes.c_int64), ("vapor_count", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 # every function include/mplbm.h declares, with its ctypes signature
 VP = ctypes.c_void_p
 SIGNATURES = {
@@ -82,6 +95,8 @@ SIGNATURES = {
     "mplbm_read_populations_f64": (ctypes.c_int, [VP, VP]),
     "mplbm_write_populations_f64": (ctypes.c_int, [VP, VP]),
     "mplbm_reduce": (ctypes.c_int, [VP, ctypes.POINTER(Stats64)]),
+    "mplbm_set_solid_numbers": (ctypes.c_int, [VP, VP, ctypes.c_int64]),
+    "mplbm_calc_cluster": (ctypes.c_int, [VP, ctypes.c_double, ctypes.c_double, ctypes.POINTER(Clusters), VP, VP]),
     "mplbm_set_tiling": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "mplbm_info_get": (ctypes.c_int, [VP, ctypes.POINTER(Info)]),
     "mplbm_version": (ctypes.c_char_p, []),

@@ -1342,6 +1342,13 @@ struct mplbm_ctx {
   CTiling ctiling{};
   std::vector<uint8_t> hmask;   // host copy of the solid mask (list rebuilds)
   size_t lat_bytes = 0;
+  // cluster statistics (allocated at the first mplbm_calc_cluster)
+  int32_t* cl_parent = nullptr;  // union-find parents, then cluster labels
+  int32_t* cl_size = nullptr;    // component sizes at roots, then cluster numbers
+  uint8_t* cl_phase = nullptr;   // 0 solid or intermediate, 1 liquid, 2 vapour
+  void* cl_acc = nullptr;        // ClAcc
+  int32_t* cl_sid = nullptr;     // solid numbers (mplbm_set_solid_numbers)
+  int64_t cl_solid_count = 0;
 };
 
 static std::string g_create_error;
@@ -1858,8 +1865,9 @@ mplbm_status mplbm_create(const mplbm_config* cfg, mplbm_ctx** out) {
 void mplbm_destroy(mplbm_ctx* ctx) {
   if (!ctx) return;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  void* bufs[] = {ctx->lat[0], ctx->lat[1], ctx->link, ctx->mask,        ctx->rho,     ctx->pres, ctx->u,
-                  ctx->psi,    ctx->red_partial, ctx->red_out, ctx->rw, ctx->cpos, ctx->ctab};
+  void* bufs[] = {ctx->lat[0], ctx->lat[1], ctx->link,     ctx->mask,      ctx->rho,      ctx->pres,
+                  ctx->u,      ctx->psi,    ctx->red_partial, ctx->red_out, ctx->rw,       ctx->cpos,
+                  ctx->ctab,   ctx->cl_parent, ctx->cl_size, ctx->cl_phase, ctx->cl_acc, ctx->cl_sid};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -2731,6 +2739,624 @@ mplbm_status mplbm_ensemble_summary(const mplbm_ensemble* e, int64_t* work_items
   if (work_items) *work_items = (int64_t)e->items.size();
   if (fluid_nodes) *fluid_nodes = e->fluid_nodes;
   if (block_threads) *block_threads = e->block;
+  return MPLBM_OK;
+}
+
+}  // extern "C"
+
+// ============================================================================
+// cluster statistics on the device (calc_cluster, src/multiphase_lbm.cu:225-562)
+//
+// Liquid (rho > rhol) and vapour (rho < rhog) fluid nodes are labelled by
+// 6-neighbour periodic connectivity with a lock-free union-find whose roots
+// are the smallest cell index of each component.  Components of fewer than 7
+// nodes are dropped and the rest numbered 1, 2, ... in the order of their
+// smallest cell index, which is the order in which the reference's scan
+// (z fastest) starts its depth-first searches, so the labels are the
+// reference's LiqClusters / GasClusters values.
+
+namespace mplbm {
+
+constexpr int kClusterMin = 7;  // smaller clusters are discarded (:330-334)
+constexpr int kClThreads = 256;
+constexpr int kClWaves = kClThreads / 64;
+
+struct ClAcc {
+  double liq_p, vap_p, liq_rho, vap_rho;
+  unsigned long long fluid, liquid, vapor, liq_solid, liq_vapor;
+  unsigned long long candidates, pairs, touching;
+  unsigned long long roots[2];
+  unsigned long long sum[3], size;
+  unsigned int rho_min, rho_max, p_min, p_max;
+  unsigned int maxr;
+};
+
+// order-preserving unsigned key of a float (min/max with integer atomics)
+__host__ __device__ inline uint32_t fkey(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__host__ __device__ inline float fkey_value(uint32_t k) {
+  const uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
+template <class T>
+__device__ __forceinline__ T cl_wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ uint32_t cl_wave_min(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return v;
+}
+__device__ __forceinline__ uint32_t cl_wave_max(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return v;
+}
+
+// block-wide sum of v added to *dst by one atomic (every thread calls it)
+template <class T>
+__device__ void cl_block_add(T v, T* dst, T* sh) {
+  v = cl_wave_sum(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    T t = 0;
+    for (int k = 0; k < kClWaves; ++k) t += sh[k];
+    if (t != (T)0) atomicAdd(dst, t);
+  }
+  __syncthreads();
+}
+template <bool kMax>
+__device__ void cl_block_minmax(uint32_t v, uint32_t* dst, uint32_t* sh) {
+  v = kMax ? cl_wave_max(v) : cl_wave_min(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = sh[0];
+    for (int k = 1; k < kClWaves; ++k) t = kMax ? max(t, sh[k]) : min(t, sh[k]);
+    if (kMax) atomicMax(dst, t);
+    else atomicMin(dst, t);
+  }
+  __syncthreads();
+}
+
+struct ClDims {
+  int lx, ly, lz;
+  uint32_t n, plane;
+  __device__ void coords(uint32_t c, int& x, int& y, int& z) const {
+    x = (int)(c / plane);
+    const uint32_t r = c - (uint32_t)x * plane;
+    y = (int)(r / (uint32_t)lz);
+    z = (int)(r - (uint32_t)y * lz);
+  }
+  __device__ uint32_t at(int x, int y, int z) const {
+    x = x < 0 ? x + lx : (x >= lx ? x - lx : x);
+    y = y < 0 ? y + ly : (y >= ly ? y - ly : y);
+    z = z < 0 ? z + lz : (z >= lz ? z - lz : z);
+    return (uint32_t)x * plane + (uint32_t)y * lz + (uint32_t)z;
+  }
+};
+
+// Phase of every cell, FP64 phase sums, extrema (write_stats starts from
+// 1e9 / -1e-9, :1058-1059) and the boundary counts of liquid nodes: the
+// first neighbour x - e_i (i = 1..18) that is solid or below rhol (:277-316).
+// Every cell starts as its own union-find root with size 0.
+template <class M>
+__global__ __launch_bounds__(kClThreads) void k_cl_classify(ClDims D, const uint8_t* __restrict__ mask,
+                                                            const M* __restrict__ rho, const M* __restrict__ pres,
+                                                            float rhol, float rhog, uint8_t* __restrict__ phase,
+                                                            int32_t* __restrict__ parent, int32_t* __restrict__ size,
+                                                            ClAcc* acc) {
+  __shared__ double shd[kClWaves];
+  __shared__ unsigned long long shu[kClWaves];
+  __shared__ uint32_t shk[kClWaves];
+  double lp = 0, vp = 0, lr = 0, vr = 0;
+  unsigned long long nf = 0, nl = 0, nv = 0, nls = 0, nlv = 0;
+  uint32_t rmin = fkey(1e9f), rmax = fkey(-1e-9f), pmin = fkey(1e9f), pmax = fkey(-1e-9f);
+  for (uint32_t c = blockIdx.x * kClThreads + threadIdx.x; c < D.n; c += gridDim.x * kClThreads) {
+    uint8_t ph = 0;
+    parent[c] = (int32_t)c;
+    size[c] = 0;
+    if (mask[c]) {
+      const float r = (float)rho[c], p = (float)pres[c];
+      ++nf;
+      rmin = min(rmin, fkey(r));
+      rmax = max(rmax, fkey(r));
+      pmin = min(pmin, fkey(p));
+      pmax = max(pmax, fkey(p));
+      if (r > rhol) {
+        ph = 1;
+        lp += (double)p;
+        lr += (double)r;
+        ++nl;
+        int x, y, z;
+        D.coords(c, x, y, z);
+        for (int i = 1; i < kQ; ++i) {
+          const uint32_t u = D.at(x - ex(i), y - ey(i), z - ez(i));
+          if (!mask[u]) {
+            ++nls;
+            break;
+          }
+          if ((float)rho[u] < rhol) {
+            ++nlv;
+            break;
+          }
+        }
+      } else if (r < rhog) {
+        ph = 2;
+        vp += (double)p;
+        vr += (double)r;
+        ++nv;
+      }
+    }
+    phase[c] = ph;
+  }
+  cl_block_add(lp, &acc->liq_p, shd);
+  cl_block_add(vp, &acc->vap_p, shd);
+  cl_block_add(lr, &acc->liq_rho, shd);
+  cl_block_add(vr, &acc->vap_rho, shd);
+  cl_block_add(nf, &acc->fluid, shu);
+  cl_block_add(nl, &acc->liquid, shu);
+  cl_block_add(nv, &acc->vapor, shu);
+  cl_block_add(nls, &acc->liq_solid, shu);
+  cl_block_add(nlv, &acc->liq_vapor, shu);
+  cl_block_minmax<false>(rmin, &acc->rho_min, shk);
+  cl_block_minmax<true>(rmax, &acc->rho_max, shk);
+  cl_block_minmax<false>(pmin, &acc->p_min, shk);
+  cl_block_minmax<true>(pmax, &acc->p_max, shk);
+}
+
+// Root of c.  Parents only ever point to smaller indices, so a stale read
+// still names an ancestor; path halving stores ancestors only.
+__device__ __forceinline__ int32_t cl_find(int32_t* parent, int32_t c) {
+  for (;;) {
+    const int32_t p = __atomic_load_n(&parent[c], __ATOMIC_RELAXED);
+    if (p == c) return c;
+    const int32_t g = __atomic_load_n(&parent[p], __ATOMIC_RELAXED);
+    if (g != p) __atomic_store_n(&parent[c], g, __ATOMIC_RELAXED);
+    c = g;
+  }
+}
+
+// root of c without path updates (once the forest no longer changes)
+__device__ __forceinline__ int32_t cl_root(const int32_t* parent, int32_t c) {
+  for (;;) {
+    const int32_t p = __atomic_load_n(&parent[c], __ATOMIC_RELAXED);
+    if (p == c) return c;
+    c = p;
+  }
+}
+
+// union: the larger root is linked under the smaller one by compare-and-swap
+__device__ __forceinline__ void cl_union(int32_t* parent, int32_t a, int32_t b) {
+  for (;;) {
+    a = cl_find(parent, a);
+    b = cl_find(parent, b);
+    if (a == b) return;
+    if (a < b) {
+      const int32_t t = a;
+      a = b;
+      b = t;
+    }
+    if (atomicCAS(&parent[a], a, b) == a) return;
+  }
+}
+
+// one edge per axis (+x, +y, +z, periodic) between nodes of the same phase
+__global__ __launch_bounds__(kClThreads) void k_cl_hook(ClDims D, const uint8_t* __restrict__ phase,
+                                                        int32_t* parent) {
+  for (uint32_t c = blockIdx.x * kClThreads + threadIdx.x; c < D.n; c += gridDim.x * kClThreads) {
+    const uint8_t ph = phase[c];
+    if (!ph) continue;
+    int x, y, z;
+    D.coords(c, x, y, z);
+    const uint32_t nb[3] = {D.at(x + 1, y, z), D.at(x, y + 1, z), D.at(x, y, z + 1)};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if (phase[nb[k]] == ph) cl_union(parent, (int32_t)c, (int32_t)nb[k]);
+  }
+}
+
+// parent := root, and component sizes counted at the roots up to
+// kClusterMin (only "fewer than 7" matters).  The only store to parent[c]
+// is c's root, so concurrent walks still see ancestors.  Lanes of a wave
+// hold consecutive cells; each run of equal roots adds its length once.
+__global__ __launch_bounds__(kClThreads) void k_cl_compress(ClDims D, const uint8_t* __restrict__ phase,
+                                                            int32_t* parent, int32_t* size) {
+  const int lane = threadIdx.x & 63;
+  for (uint32_t base = blockIdx.x * kClThreads; base < D.n; base += gridDim.x * kClThreads) {
+    const uint32_t c = base + threadIdx.x;
+    int32_t r = -1;
+    if (c < D.n && phase[c]) r = cl_root(parent, (int32_t)c);
+    const int32_t prev = __shfl_up(r, 1, 64);
+    const bool start = lane == 0 || prev != r;
+    const unsigned long long starts = __ballot(start);
+    if (c < D.n && phase[c]) parent[c] = r;
+    if (start && r >= 0) {
+      const unsigned long long above = starts & ~((2ull << lane) - 1ull);
+      const int next = above ? (int)__ffsll((long long)above) - 1 : 64;
+      if (__atomic_load_n(&size[r], __ATOMIC_RELAXED) < kClusterMin) atomicAdd(&size[r], next - lane);
+    }
+  }
+}
+
+// Roots of kept components (kClusterMin nodes or more): counted (lists
+// null) or appended to the per-phase lists, dropped roots' size set to 0.
+__global__ __launch_bounds__(kClThreads) void k_cl_roots(ClDims D, const uint8_t* __restrict__ phase,
+                                                         const int32_t* __restrict__ parent, int32_t* size,
+                                                         int32_t* list_liq, int32_t* list_gas, ClAcc* acc) {
+  for (uint32_t c = blockIdx.x * kClThreads + threadIdx.x; c < D.n; c += gridDim.x * kClThreads) {
+    const uint8_t ph = phase[c];
+    if (!ph || parent[c] != (int32_t)c) continue;
+    if (size[c] >= kClusterMin) {
+      const unsigned long long k = atomicAdd(&acc->roots[ph - 1], 1ull);
+      if (list_liq) (ph == 1 ? list_liq : list_gas)[k] = (int32_t)c;
+    } else if (list_liq) {
+      size[c] = 0;
+    }
+  }
+}
+
+__global__ void k_cl_assign(const int32_t* __restrict__ list, int32_t count, int32_t* size) {
+  const int32_t k = blockIdx.x * kClThreads + threadIdx.x;
+  if (k < count) size[list[k]] = k + 1;
+}
+
+// label := cluster number of the node's root (0: dropped or no phase)
+__global__ __launch_bounds__(kClThreads) void k_cl_label(ClDims D, const uint8_t* __restrict__ phase,
+                                                         int32_t* parent, const int32_t* __restrict__ size) {
+  for (uint32_t c = blockIdx.x * kClThreads + threadIdx.x; c < D.n; c += gridDim.x * kClThreads)
+    parent[c] = phase[c] ? size[parent[c]] : 0;
+}
+
+// Distinct (liquid cluster, solid number) pairs over the 6 neighbours of
+// the nodes of kept liquid clusters (the solids_touched sets of :318-360).
+// table null: count the candidate incidences only.
+__global__ __launch_bounds__(kClThreads) void k_cl_pairs(ClDims D, const uint8_t* __restrict__ phase,
+                                                         const int32_t* __restrict__ label,
+                                                         const int32_t* __restrict__ sid,
+                                                         unsigned long long* table, unsigned long long mask,
+                                                         int32_t* touch, ClAcc* acc) {
+  __shared__ unsigned long long shu[kClWaves];
+  unsigned long long cand = 0, pairs = 0, touching = 0;
+  for (uint32_t c = blockIdx.x * kClThreads + threadIdx.x; c < D.n; c += gridDim.x * kClThreads) {
+    if (phase[c] != 1) continue;
+    const int32_t lab = label[c];
+    if (lab <= 0) continue;
+    int x, y, z;
+    D.coords(c, x, y, z);
+    const uint32_t nb[6] = {D.at(x - 1, y, z), D.at(x, y - 1, z), D.at(x, y, z - 1),
+                            D.at(x + 1, y, z), D.at(x, y + 1, z), D.at(x, y, z + 1)};
+    for (int k = 0; k < 6; ++k) {
+      const int32_t s = sid[nb[k]];
+      if (s == 0 || phase[nb[k]] == 1) continue;
+      if (!table) {
+        ++cand;
+        continue;
+      }
+      const unsigned long long key = ((unsigned long long)(uint32_t)lab << 32) | (uint32_t)s;
+      unsigned long long h = (key * 0x9E3779B97F4A7C15ull) >> 17;
+      for (;;) {
+        h &= mask;
+        const unsigned long long cur = __atomic_load_n(&table[h], __ATOMIC_RELAXED);
+        if (cur == key) break;
+        if (cur == ~0ull) {
+          const unsigned long long old = atomicCAS(&table[h], ~0ull, key);
+          if (old == ~0ull) {
+            ++pairs;
+            if (atomicExch(&touch[lab], 1) == 0) ++touching;
+            break;
+          }
+          if (old == key) break;
+        }
+        ++h;
+      }
+    }
+  }
+  cl_block_add(cand, &acc->candidates, shu);
+  cl_block_add(pairs, &acc->pairs, shu);
+  cl_block_add(touching, &acc->touching, shu);
+}
+
+// Single liquid cluster (:362-533): whether it touches both faces of an
+// axis (brk) and which planes along each axis it occupies.
+__global__ __launch_bounds__(kClThreads) void k_cl_span(ClDims D, const uint8_t* __restrict__ phase,
+                                                        const int32_t* __restrict__ label, int32_t* occ,
+                                                        int32_t* brk) {
+  for (uint32_t c = blockIdx.x * kClThreads + threadIdx.x; c < D.n; c += gridDim.x * kClThreads) {
+    if (phase[c] != 1 || label[c] != 1) continue;
+    int x, y, z;
+    D.coords(c, x, y, z);
+    occ[x] = 1;
+    occ[D.lx + y] = 1;
+    occ[D.lx + D.ly + z] = 1;
+    auto one = [&](uint32_t u) { return phase[u] == 1 && label[u] == 1; };
+    if (x == 0 && one(D.at(D.lx - 1, y, z))) brk[0] = 1;
+    if (y == 0 && one(D.at(x, D.ly - 1, z))) brk[1] = 1;
+    if (z == 0 && one(D.at(x, y, D.lz - 1))) brk[2] = 1;
+  }
+}
+
+struct ClShift {
+  int brk[3], vplane[3];
+  float ctr[3];
+};
+
+// coordinate of a cluster node along axis a, unwrapped across the first
+// free plane of a broken axis
+__device__ __forceinline__ int cl_unwrap(const ClShift& S, const ClDims& D, int a, int v) {
+  const int L = a == 0 ? D.lx : (a == 1 ? D.ly : D.lz);
+  return v + ((S.brk[a] && v < S.vplane[a]) ? L : 0);
+}
+
+// centre sums (pass 0) or the largest distance from the centre (pass 1)
+__global__ __launch_bounds__(kClThreads) void k_cl_shape(ClDims D, const uint8_t* __restrict__ phase,
+                                                         const int32_t* __restrict__ label, ClShift S, int pass,
+                                                         ClAcc* acc) {
+  __shared__ unsigned long long shu[kClWaves];
+  __shared__ uint32_t shk[kClWaves];
+  unsigned long long s[3] = {0, 0, 0}, cnt = 0;
+  float maxr = 0.f;
+  for (uint32_t c = blockIdx.x * kClThreads + threadIdx.x; c < D.n; c += gridDim.x * kClThreads) {
+    if (phase[c] != 1 || label[c] != 1) continue;
+    int co[3];
+    D.coords(c, co[0], co[1], co[2]);
+    if (pass == 0) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) s[a] += (unsigned long long)cl_unwrap(S, D, a, co[a]);
+      ++cnt;
+    } else {
+      float r2 = 0.f;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const float adj = (float)cl_unwrap(S, D, a, co[a]);
+        r2 += (S.ctr[a] - adj) * (S.ctr[a] - adj);
+      }
+      maxr = max(maxr, __builtin_sqrtf(r2));
+    }
+  }
+  if (pass == 0) {
+    cl_block_add(s[0], &acc->sum[0], shu);
+    cl_block_add(s[1], &acc->sum[1], shu);
+    cl_block_add(s[2], &acc->sum[2], shu);
+    cl_block_add(cnt, &acc->size, shu);
+  } else {
+    cl_block_minmax<true>(__float_as_uint(maxr), &acc->maxr, shk);
+  }
+}
+
+}  // namespace mplbm
+
+// ----------------------------------------------------------------------------
+// cluster statistics API
+
+static unsigned cl_blocks(uint64_t n) {
+  const uint64_t b = (n + kClThreads - 1) / kClThreads;
+  return (unsigned)std::min<uint64_t>(std::max<uint64_t>(b, 1), 8192);
+}
+
+static mplbm_status cl_read_acc(mplbm_ctx* ctx, ClAcc* h) {
+  HIP_TRY(ctx, hipMemcpyAsync(h, ctx->cl_acc, sizeof(ClAcc), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return MPLBM_OK;
+}
+
+// kept roots of one phase: sorted ascending and numbered 1, 2, ... at the roots
+static mplbm_status cl_number(mplbm_ctx* ctx, int32_t* d_list, int64_t count) {
+  if (count == 0) return MPLBM_OK;
+  std::vector<int32_t> h((size_t)count);
+  HIP_TRY(ctx, hipMemcpyAsync(h.data(), d_list, (size_t)count * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  std::sort(h.begin(), h.end());
+  HIP_TRY(ctx, hipMemcpyAsync(d_list, h.data(), (size_t)count * 4, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_cl_assign, dim3((unsigned)((count + kClThreads - 1) / kClThreads)), dim3(kClThreads), 0,
+                     ctx->stream, d_list, (int32_t)count, ctx->cl_size);
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // h is released on return
+  return MPLBM_OK;
+}
+
+extern "C" {
+
+mplbm_status mplbm_set_solid_numbers(mplbm_ctx* ctx, const int64_t* solid_number, int64_t solid_count) {
+  if (!ctx) return fail(ctx, MPLBM_ERR_ARG, "null context");
+  if (solid_count < 0) return fail(ctx, MPLBM_ERR_ARG, "negative solid_count");
+  const uint64_t n = ctx->ncell;
+  if (!solid_number) {
+    if (ctx->cl_sid) HIP_TRY(ctx, hipFree(ctx->cl_sid));
+    ctx->cl_sid = nullptr;
+    ctx->cl_solid_count = solid_count;
+    return MPLBM_OK;
+  }
+  std::vector<int32_t> h(n);
+  for (uint64_t c = 0; c < n; ++c) {
+    if (solid_number[c] < 0 || solid_number[c] > INT32_MAX)
+      return fail(ctx, MPLBM_ERR_ARG, "solid number %lld at cell %llu outside [0, 2^31)",
+                  (long long)solid_number[c], (unsigned long long)c);
+    h[c] = (int32_t)solid_number[c];
+  }
+  if (!ctx->cl_sid) HIP_TRY(ctx, hipMalloc((void**)&ctx->cl_sid, n * 4));
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->cl_sid, h.data(), n * 4, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->cl_solid_count = solid_count;
+  return MPLBM_OK;
+}
+
+mplbm_status mplbm_calc_cluster(mplbm_ctx* ctx, double rhol_spinodal, double rhog_spinodal, mplbm_clusters* out,
+                                 int32_t* liq_label, int32_t* gas_label) {
+  if (!ctx || !out) return fail(ctx, MPLBM_ERR_ARG, "null argument");
+  if (!ctx->have_fields) return fail(ctx, MPLBM_ERR_STATE, "fields not initialised");
+  const uint64_t n = ctx->ncell;
+  if (n >= (1ull << 31)) return fail(ctx, MPLBM_ERR_UNSUPPORTED, "cluster labels need fewer than 2^31 cells");
+  if (!ctx->cl_parent) {
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->cl_parent, n * 4));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->cl_size, n * 4));
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->cl_phase, n));
+    HIP_TRY(ctx, hipMalloc(&ctx->cl_acc, sizeof(ClAcc)));
+  }
+  const ClDims D{(int)ctx->cfg.lx, (int)ctx->cfg.ly, (int)ctx->cfg.lz, (uint32_t)n,
+                 (uint32_t)(ctx->cfg.ly * ctx->cfg.lz)};
+  const unsigned nb = cl_blocks(n);
+  const dim3 blk(kClThreads);
+  ClAcc h{};
+  h.rho_min = fkey(1e9f);
+  h.rho_max = fkey(-1e-9f);
+  h.p_min = fkey(1e9f);
+  h.p_max = fkey(-1e-9f);
+  ClAcc* acc = reinterpret_cast<ClAcc*>(ctx->cl_acc);
+  HIP_TRY(ctx, hipMemcpyAsync(acc, &h, sizeof h, hipMemcpyHostToDevice, ctx->stream));
+  // thresholds compare as the reference's floats (real rho > rhol_spinodal)
+  const float rhol = (float)rhol_spinodal, rhog = (float)rhog_spinodal;
+  if (ctx->cfg.storage == MPLBM_FP64)
+    hipLaunchKernelGGL(k_cl_classify<double>, dim3(nb), blk, 0, ctx->stream, D, ctx->mask,
+                       (const double*)ctx->rho, (const double*)ctx->pres, rhol, rhog, ctx->cl_phase, ctx->cl_parent,
+                       ctx->cl_size, acc);
+  else
+    hipLaunchKernelGGL(k_cl_classify<float>, dim3(nb), blk, 0, ctx->stream, D, ctx->mask, (const float*)ctx->rho,
+                       (const float*)ctx->pres, rhol, rhog, ctx->cl_phase, ctx->cl_parent, ctx->cl_size, acc);
+  hipLaunchKernelGGL(k_cl_hook, dim3(nb), blk, 0, ctx->stream, D, ctx->cl_phase, ctx->cl_parent);
+  hipLaunchKernelGGL(k_cl_compress, dim3(nb), blk, 0, ctx->stream, D, ctx->cl_phase, ctx->cl_parent, ctx->cl_size);
+  hipLaunchKernelGGL(k_cl_roots, dim3(nb), blk, 0, ctx->stream, D, ctx->cl_phase, ctx->cl_parent, ctx->cl_size,
+                     (int32_t*)nullptr, (int32_t*)nullptr, acc);
+  HIP_TRY(ctx, hipGetLastError());
+  mplbm_status st = cl_read_acc(ctx, &h);
+  if (st != MPLBM_OK) return st;
+  const int64_t nroot[2] = {(int64_t)h.roots[0], (int64_t)h.roots[1]};
+  int32_t* lists = nullptr;
+  HIP_TRY(ctx, hipMalloc((void**)&lists, (size_t)(nroot[0] + nroot[1] + 1) * 4));
+  auto run = [&]() -> mplbm_status {
+    const unsigned long long zero[2] = {0, 0};
+    HIP_TRY(ctx, hipMemcpyAsync(&acc->roots[0], zero, sizeof zero, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_cl_roots, dim3(nb), blk, 0, ctx->stream, D, ctx->cl_phase, ctx->cl_parent, ctx->cl_size,
+                       lists, lists + nroot[0], acc);
+    HIP_TRY(ctx, hipGetLastError());
+    mplbm_status s2 = cl_number(ctx, lists, nroot[0]);
+    if (s2 == MPLBM_OK) s2 = cl_number(ctx, lists + nroot[0], nroot[1]);
+    return s2;
+  };
+  st = run();
+  (void)hipFree(lists);
+  if (st != MPLBM_OK) return st;
+  hipLaunchKernelGGL(k_cl_label, dim3(nb), blk, 0, ctx->stream, D, ctx->cl_phase, ctx->cl_parent, ctx->cl_size);
+  HIP_TRY(ctx, hipGetLastError());
+
+  // cluster order and wet coordination number (:535-561): distinct
+  // (cluster, solid) pairs in an open-addressing table
+  if (ctx->cl_sid && nroot[0] > 0) {
+    hipLaunchKernelGGL(k_cl_pairs, dim3(nb), blk, 0, ctx->stream, D, ctx->cl_phase, ctx->cl_parent, ctx->cl_sid,
+                       (unsigned long long*)nullptr, 0ull, (int32_t*)nullptr, acc);
+    HIP_TRY(ctx, hipGetLastError());
+    if ((st = cl_read_acc(ctx, &h)) != MPLBM_OK) return st;
+    if (h.candidates > 0) {
+      unsigned long long cap = 1024;
+      while (cap < 2 * h.candidates) cap <<= 1;
+      void* tab = nullptr;
+      HIP_TRY(ctx, hipMalloc(&tab, cap * 8 + (size_t)(nroot[0] + 1) * 4));
+      int32_t* touch = reinterpret_cast<int32_t*>(reinterpret_cast<unsigned long long*>(tab) + cap);
+      hipError_t e = hipMemsetAsync(tab, 0xFF, cap * 8, ctx->stream);
+      if (e == hipSuccess) e = hipMemsetAsync(touch, 0, (size_t)(nroot[0] + 1) * 4, ctx->stream);
+      if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_cl_pairs, dim3(nb), blk, 0, ctx->stream, D, ctx->cl_phase, ctx->cl_parent,
+                           ctx->cl_sid, (unsigned long long*)tab, cap - 1, touch, acc);
+        e = hipGetLastError();
+      }
+      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+      (void)hipFree(tab);
+      if (e != hipSuccess) return fail(ctx, MPLBM_ERR_HIP, "cluster pairs: %s", hipGetErrorString(e));
+    }
+  }
+
+  // single liquid cluster: periodic unwrapping, centre and radius (:362-533)
+  double radius = 0.0;
+  if (nroot[0] == 1) {
+    const int64_t L[3] = {ctx->cfg.lx, ctx->cfg.ly, ctx->cfg.lz};
+    const size_t nocc = (size_t)(L[0] + L[1] + L[2]);
+    int32_t* occ = nullptr;
+    HIP_TRY(ctx, hipMalloc((void**)&occ, (nocc + 3) * 4));
+    std::vector<int32_t> ho(nocc + 3);
+    hipError_t e = hipMemsetAsync(occ, 0, (nocc + 3) * 4, ctx->stream);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_cl_span, dim3(nb), blk, 0, ctx->stream, D, ctx->cl_phase, ctx->cl_parent, occ,
+                         occ + nocc);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(ho.data(), occ, (nocc + 3) * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(occ);
+    if (e != hipSuccess) return fail(ctx, MPLBM_ERR_HIP, "cluster span: %s", hipGetErrorString(e));
+    ClShift S{};
+    int64_t off = 0;
+    for (int a = 0; a < 3; ++a) {
+      S.brk[a] = ho[nocc + a] != 0;
+      S.vplane[a] = 0;
+      if (S.brk[a])  // first plane free of the cluster
+        for (int64_t k = 0; k < L[a]; ++k)
+          if (!ho[(size_t)(off + k)]) {
+            S.vplane[a] = (int)k;
+            break;
+          }
+      off += L[a];
+    }
+    hipLaunchKernelGGL(k_cl_shape, dim3(nb), blk, 0, ctx->stream, D, ctx->cl_phase, ctx->cl_parent, S, 0, acc);
+    HIP_TRY(ctx, hipGetLastError());
+    if ((st = cl_read_acc(ctx, &h)) != MPLBM_OK) return st;
+    // integer division of the sums as in the reference (size_t sums, :496-498)
+    for (int a = 0; a < 3; ++a) S.ctr[a] = h.size ? (float)(h.sum[a] / h.size) : 0.f;
+    hipLaunchKernelGGL(k_cl_shape, dim3(nb), blk, 0, ctx->stream, D, ctx->cl_phase, ctx->cl_parent, S, 1, acc);
+    HIP_TRY(ctx, hipGetLastError());
+    if ((st = cl_read_acc(ctx, &h)) != MPLBM_OK) return st;
+    float r;
+    memcpy(&r, &h.maxr, 4);
+    radius = r;
+  }
+  if ((st = cl_read_acc(ctx, &h)) != MPLBM_OK) return st;
+
+  if (liq_label || gas_label) {
+    std::vector<int32_t> lab(n);
+    std::vector<uint8_t> ph(n);
+    HIP_TRY(ctx, hipMemcpyAsync(lab.data(), ctx->cl_parent, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(ph.data(), ctx->cl_phase, n, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    for (uint64_t c = 0; c < n; ++c) {
+      if (liq_label) liq_label[c] = ph[c] == 1 ? lab[c] : 0;
+      if (gas_label) gas_label[c] = ph[c] == 2 ? lab[c] : 0;
+    }
+  }
+
+  mplbm_clusters o{};
+  o.min_rho = fkey_value(h.rho_min);
+  o.max_rho = fkey_value(h.rho_max);
+  o.min_p = fkey_value(h.p_min);
+  o.max_p = fkey_value(h.p_max);
+  o.fluid_count = (int64_t)h.fluid;
+  o.liquid_count = (int64_t)h.liquid;
+  o.vapor_count = (int64_t)h.vapor;
+  if (h.liquid) {
+    o.avg_liq_press = (float)(h.liq_p / (double)h.liquid);
+    o.avg_liq_rho = (float)(h.liq_rho / (double)h.liquid);
+    o.ratio_liquid_solid_boundary = (float)((double)h.liq_solid / (double)h.liquid);
+    o.ratio_liquid_vapor_boundary = (float)((double)h.liq_vapor / (double)h.liquid);
+  }
+  if (h.vapor) {
+    o.avg_vap_press = (float)(h.vap_p / (double)h.vapor);
+    o.avg_vap_rho = (float)(h.vap_rho / (double)h.vapor);
+  }
+  o.saturation = h.fluid ? (double)((float)h.liquid / (float)h.fluid) : 0.0;
+  o.liq_cluster_count = nroot[0];
+  o.gas_cluster_count = nroot[1];
+  // sums of set sizes in float as the reference accumulates them
+  o.cluster_order = h.touching ? (double)((float)h.pairs / (float)h.touching) : 0.0;
+  o.wet_coord_num = ctx->cl_solid_count ? (double)((float)h.pairs / (float)ctx->cl_solid_count) : (double)(float)h.pairs;
+  o.cluster_radius = radius;
+  *out = o;
   return MPLBM_OK;
 }
 
