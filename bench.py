@@ -65,6 +65,10 @@ def parse():
                     help="768x512x512 packing between a liquid inlet and a vapour outlet: FP32 vs FP16 throughput, "
                          "and FP64/FP32/FP16 saturation on a 1/8 sub-volume")
     ap.add_argument("--drainage-steps", type=int, default=20000)
+    ap.add_argument("--clusters", action="store_true",
+                    help="calc_cluster statistics of the 512^3 two-phase run: device (mplbm_calc_cluster) against "
+                         "the host path (field read + host cluster_stats), same results")
+    ap.add_argument("--cluster-steps", type=int, default=2000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain device-to-device copy kernel of 2 GiB (rocprofv3 --pmc calibration)")
     return ap.parse_args()
@@ -78,7 +82,7 @@ def workload(a):
     rho = H.rho_init(shape, 0.7, 0.02 / 0.7)
     print(f"# workload {shape}: {len(sph)} spheres, porosity {fluid.mean():.4f}, built in {time.time() - t:.1f}s",
           file=sys.stderr, flush=True)
-    return shape, fluid, rho, len(sph)
+    return shape, fluid, rho, sph
 
 
 PHYS = dict(tau=1.0, G=-5.0, rhos=0.7, rhol_spinodal=1.285931, rhog_spinodal=0.323507)
@@ -396,6 +400,51 @@ def drainage_bench(a):
     print(json.dumps(out), flush=True)
 
 
+def cluster_bench(a, shape, fluid, rho, sph):
+    """Statistics step of the driver (calc_cluster, src/multiphase_lbm.cu:225-562) on the workload after
+    --cluster-steps two-phase steps: device union-find labelling against the reference's way (read rho and p
+    to the host, then the host cluster search)."""
+    import multiphase_lbm_amd as mp
+    from multiphase_lbm_amd import harness as H
+    m2 = np.ones(shape, np.uint8)
+    sn = np.zeros(shape, np.int64)
+    for k, (x, y, z, r) in enumerate(sph):
+        H.place_sphere(m2, float(x), float(y), float(z), float(r), sn, k + 1)
+    assert np.array_equal(m2, fluid.astype(np.uint8)), "solid numbering does not reproduce the packing"
+    with mp.Context(*shape, storage="fp32", kernel=a.kernel, **PHYS) as ctx:
+        ctx.set_geometry(fluid)
+        ctx.set_solid_numbers(sn, len(sph))
+        ctx.init_fields(rho)
+        ctx.step(0, a.cluster_steps, write_macro=True)
+        ctx.sync()
+        print(f"# clusters: {a.cluster_steps} steps done", file=sys.stderr, flush=True)
+        ctx.calc_cluster(PHYS["rhol_spinodal"], PHYS["rhog_spinodal"])  # first call allocates
+        times = []
+        for _ in range(3):
+            t = time.perf_counter()
+            dev = ctx.calc_cluster(PHYS["rhol_spinodal"], PHYS["rhog_spinodal"])
+            times.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        r, p, _ = ctx.read_fields(np.float32)
+        t_read = time.perf_counter() - t
+    print(f"# clusters: device {sorted(times)[1] * 1e3:.2f} ms, field read {t_read:.2f} s; host search next",
+          file=sys.stderr, flush=True)
+    t = time.perf_counter()
+    host = H.cluster_stats(fluid, r, p, sn, PHYS["rhol_spinodal"], PHYS["rhog_spinodal"], len(sph))
+    t_host = time.perf_counter() - t
+    exact = [k for k in dev if k not in ("avg_liq_press", "avg_vap_press", "avg_liq_rho", "avg_vap_rho")]
+    same = all(dev[k] == host[k] for k in exact) and all(
+        abs(dev[k] - host[k]) <= 1e-6 * max(1e-30, abs(host[k])) for k in dev if k not in exact)
+    dev_ms = sorted(times)[1] * 1e3
+    print(json.dumps({"mode": "clusters", "workload": f"{shape[0]}^3 packing after {a.cluster_steps} steps, G=-5",
+                      "device_ms": round(dev_ms, 2), "host_read_fields_s": round(t_read, 3),
+                      "host_cluster_stats_s": round(t_host, 3),
+                      "speedup_vs_host_path": round((t_read + t_host) / (dev_ms / 1e3), 1),
+                      "results_equal": bool(same), "liq_clusters": dev["liq_cluster_count"],
+                      "gas_clusters": dev["gas_cluster_count"], "saturation": dev["saturation"],
+                      "cluster_order": dev["cluster_order"], "wet_coord_num": dev["wet_coord_num"]}), flush=True)
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -411,7 +460,11 @@ def main():
     if a.drainage:
         drainage_bench(a)
         return
-    shape, fluid, rho, nsph = workload(a)
+    shape, fluid, rho, sph = workload(a)
+    nsph = len(sph)
+    if a.clusters:
+        cluster_bench(a, shape, fluid, rho, sph)
+        return
     if a.precision:
         precision_study(a, shape, fluid, rho)
         return

@@ -50,6 +50,8 @@ typedef struct mplbm_input {
   double face_rho[6];
   double body_force[3];
   int32_t unknown_tokens;   /* tokens reported as undefined */
+  int32_t host_statistics;  /* 1: calc_cluster on the host after a field read
+                               (the reference's way); 0: mplbm_calc_cluster */
 } mplbm_input;
 
 /* Statistics of MultiphaseLBM::calc_cluster (src/multiphase_lbm.cu:225-562). */

@@ -569,6 +569,11 @@ struct MultiphaseLBM {
       mplbm_destroy(ctx);
       return -1;
     }
+    if (!in.host_statistics && !check(mplbm_set_solid_numbers(ctx, solid_number.data(), solid_count))) {
+      mplbm_destroy(ctx);
+      return -1;
+    }
+    std::vector<int32_t> liq32, gas32;
     const int reverse = 1;
     const int64_t stat_period = in.statPeriod > 0 ? in.statPeriod : 1;
     size_t vtk_num = 0;
@@ -588,14 +593,44 @@ struct MultiphaseLBM {
       }
       step = last;
       if (stat) {
-        if (!check(mplbm_read_fields(ctx, rho.data(), pres.data(), nullptr))) { rc = -1; break; }
+        const bool vtk = in.save_vtk_ascii && in.vtkPeriod > 0 && step % in.vtkPeriod == 0;
         mplbm_cluster_stats s{};
-        cluster_stats(d, is_fluid.data(), rho.data(), pres.data(), solid_number.data(), (float)in.rhol_spinodal,
-                      (float)in.rhog_spinodal, solid_count, liq_label.data(), gas_label.data(), &s);
+        if (in.host_statistics) {
+          if (!check(mplbm_read_fields(ctx, rho.data(), pres.data(), nullptr))) { rc = -1; break; }
+          cluster_stats(d, is_fluid.data(), rho.data(), pres.data(), solid_number.data(), (float)in.rhol_spinodal,
+                        (float)in.rhog_spinodal, solid_count, liq_label.data(), gas_label.data(), &s);
+        } else {
+          // statistics on the device; fields and labels cross to the host
+          // only for a VTK file
+          if (vtk) {
+            liq32.resize(d.n());
+            gas32.resize(d.n());
+          }
+          mplbm_clusters g{};
+          if (!check(mplbm_calc_cluster(ctx, (float)in.rhol_spinodal, (float)in.rhog_spinodal, &g,
+                                        vtk ? liq32.data() : nullptr, vtk ? gas32.data() : nullptr))) {
+            rc = -1;
+            break;
+          }
+          s.min_rho = g.min_rho; s.max_rho = g.max_rho; s.min_p = g.min_p; s.max_p = g.max_p;
+          s.avg_liq_press = g.avg_liq_press; s.avg_vap_press = g.avg_vap_press; s.saturation = g.saturation;
+          s.avg_liq_rho = g.avg_liq_rho; s.avg_vap_rho = g.avg_vap_rho;
+          s.liq_cluster_count = g.liq_cluster_count; s.gas_cluster_count = g.gas_cluster_count;
+          s.cluster_order = g.cluster_order; s.wet_coord_num = g.wet_coord_num;
+          s.ratio_liquid_solid_boundary = g.ratio_liquid_solid_boundary;
+          s.ratio_liquid_vapor_boundary = g.ratio_liquid_vapor_boundary;
+          s.cluster_radius = g.cluster_radius;
+          s.fluid_count = g.fluid_count; s.liquid_count = g.liquid_count; s.vapor_count = g.vapor_count;
+          if (vtk) {
+            if (!check(mplbm_read_fields(ctx, rho.data(), pres.data(), nullptr))) { rc = -1; break; }
+            std::copy(liq32.begin(), liq32.end(), liq_label.begin());
+            std::copy(gas32.begin(), gas32.end(), gas_label.begin());
+          }
+        }
         std::printf("saturation=%f, aLp=%f,aVp=%f, step=%ld\n", s.saturation, s.avg_liq_press, s.avg_vap_press,
                     (long)step);
         write_stats_row(step, s);
-        if (in.save_vtk_ascii && in.vtkPeriod > 0 && step % in.vtkPeriod == 0) {
+        if (vtk) {
           std::cout << "Saving vtk ascii file number " << vtk_num << std::endl;
           char name[512];
           std::snprintf(name, sizeof name, "%s/result%06zu.vtk", in.result_folder, vtk_num);
@@ -693,6 +728,7 @@ bool MultiphaseLBM::mplbm_parse_scalar_token(std::istream& file, const std::stri
     in.face_rho[2 * a] = lo;
     in.face_rho[2 * a + 1] = hi;
   } else if (token == "body_force") file >> in.body_force[0] >> in.body_force[1] >> in.body_force[2];
+  else if (token == "host_statistics") file >> in.host_statistics;
   else return false;
   return true;
 }

@@ -38,6 +38,7 @@ class Input(ctypes.Structure):
         ("storage", ctypes.c_int32), ("kernel", ctypes.c_int32), ("math", ctypes.c_int32),
         ("face_type", ctypes.c_int32 * 6), ("face_rho", ctypes.c_double * 6),
         ("body_force", ctypes.c_double * 3), ("unknown_tokens", ctypes.c_int32),
+        ("host_statistics", ctypes.c_int32),
     ]
 
     def as_dict(self):
