@@ -135,6 +135,46 @@ def test_compact_tilings_agree(tiles):
     assert np.array_equal(g_rho[fluid], st.rho[fluid].astype(np.float64))
 
 
+@pytest.mark.parametrize("shape,tiles", [((7, 10, 64), (4, 32, 3, 0)), ((6, 9, 64), (5, 64, 4, 0)),
+                                         ((5, 11, 96), (3, 16, 2, 0)), ((5, 7, 96), (4, 48, 5, 256)),
+                                         ((4, 13, 64), (2, 20, 8, 0)), ((1, 5, 64), (8, 32, 1, 0))])
+def test_compact_tiles_across_z_blocks(shape, tiles):
+    """lz a multiple of 32 (several 32-wide rank blocks per row): tiles on
+    block edges (32, 64 wide), inside blocks (16, 20) and across them (48),
+    partial tiles, one plane."""
+    ty, tz, seg, nt = tiles
+    fluid, rho = _case(shape, 5, 1.5, 3.5, seed=sum(shape))
+    print(f"case compact z blocks {shape} {tiles}", flush=True)
+    st, (g_rho, g_p, g_u, g_f, _) = _run_pair(shape, fluid, rho, 10, "fp32", "compact", tile_y=ty, tile_z=tz,
+                                              seg_x=seg, block_threads=nt, **SC)
+    assert np.array_equal(g_f[fluid], st.fA[fluid].astype(np.float64))
+    assert np.array_equal(g_rho[fluid], st.rho[fluid].astype(np.float64))
+
+
+def test_compact_z_faces_and_restart():
+    """Pressure faces on z (the face columns are rank-block edges) and a
+    population write/read round trip in the middle of the run."""
+    shape = (6, 10, 64)
+    fluid, rho = _case(shape, 4, 1.5, 3.0, seed=31)
+    params = dict(SC, G=-3.0, face_type=(0, 0, 0, 0, 1, 1), face_rho=(0, 0, 0, 0, 1.04, 0.96))
+    print(f"case compact z faces {shape}", flush=True)
+    st, (g_rho, g_p, g_u, g_f, _) = _run_pair(shape, fluid, rho, 8, "fp32", "compact", **params)
+    assert np.isfinite(g_f[fluid]).all()
+    assert np.array_equal(g_f[fluid], st.fA[fluid].astype(np.float64))
+    P = oracle.Params(*shape, **params)
+    ref = oracle.State(P, fluid, rho.astype(np.float32), dtype=np.float32)
+    ref.run(4)
+    with mp.Context(*shape, storage="fp32", kernel="compact", **params) as ctx:
+        ctx.set_geometry(fluid)
+        ctx.init_fields(rho.astype(np.float32))
+        ctx.step(0, 2)
+        f = ctx.read_populations()
+        ctx.write_populations(f)
+        ctx.step(2, 2, write_macro=True)
+        g = ctx.read_populations()
+    assert np.array_equal(g[fluid], ref.fA[fluid].astype(np.float64))
+
+
 @pytest.mark.parametrize("shape", [(5, 2, 3), (2, 3, 40), (1, 6, 5), (3, 1, 1)])
 def test_compact_tiny_domains(shape):
     """Axes shorter than the tile and its rings (rows and columns of the LDS
