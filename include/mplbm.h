@@ -223,6 +223,17 @@ mplbm_status mplbm_read_fields_f64(mplbm_ctx* ctx, double* rho, double* pressure
 mplbm_status mplbm_read_populations_f64(mplbm_ctx* ctx, double* f);
 mplbm_status mplbm_write_populations_f64(mplbm_ctx* ctx, const double* f);
 
+/* Exact checkpoint (the reference can only restart from a density VTK,
+ * src/multiphase_lbm.cu:112-132): the stored values of the newest
+ * populations of every fluid node ([x][y][z] order, 19 each, in the
+ * context's storage form, with its deviation reference and scale), the step
+ * number and the domain, storage and geometry they belong to.  A loaded
+ * context continues bit for bit as the saved one would have.  Loading checks
+ * the domain, storage and fluid mask (set the geometry first) and returns
+ * the saved step in *step (may be NULL).  Synchronise. */
+mplbm_status mplbm_save_checkpoint(mplbm_ctx* ctx, const char* path, int64_t step);
+mplbm_status mplbm_load_checkpoint(mplbm_ctx* ctx, const char* path, int64_t* step);
+
 /* FP64 reductions of the newest lattice and of the last macro write
  * (synchronises). */
 mplbm_status mplbm_reduce(mplbm_ctx* ctx, mplbm_stats64* out);

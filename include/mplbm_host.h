@@ -52,6 +52,9 @@ typedef struct mplbm_input {
   int32_t unknown_tokens;   /* tokens reported as undefined */
   int32_t host_statistics;  /* 1: calc_cluster on the host after a field read
                                (the reference's way); 0: mplbm_calc_cluster */
+  int32_t read_checkpoint;  /* 1: resume from checkpoint_filename (token read_checkpoint) */
+  int64_t checkpointPeriod; /* exact checkpoint every N steps to <result_folder>/checkpoint.mplbm */
+  char checkpoint_filename[256];
 } mplbm_input;
 
 /* Statistics of MultiphaseLBM::calc_cluster (src/multiphase_lbm.cu:225-562). */
@@ -122,6 +125,16 @@ int mplbm_host_cluster_stats(const uint8_t* is_fluid, const float* rho, const fl
 
 /* write_vtk_ascii (src/multiphase_lbm.cu:948-1054), same format strings. */
 int mplbm_host_write_vtk(const char* path, const uint8_t* is_fluid, const float* rho, const float* pres,
+                         const int64_t* liq_label, const int64_t* gas_label, const int64_t* solid_number,
+                         int64_t lx, int64_t ly, int64_t lz);
+
+/* Binary counterpart of write_vtk for the vtrPeriod token (parsed by the
+ * reference, :645-650, with no writer): VTK XML RectilinearGrid, raw
+ * appended little-endian data with UInt64 block sizes, the same point arrays
+ * (Zones, Density and Pressure as Float32 with 0 at solids, LiqClusters,
+ * GasClusters, Solids as Int64) in the same x-fastest point order and the
+ * same coordinates i/lx. */
+int mplbm_host_write_vtr(const char* path, const uint8_t* is_fluid, const float* rho, const float* pres,
                          const int64_t* liq_label, const int64_t* gas_label, const int64_t* solid_number,
                          int64_t lx, int64_t ly, int64_t lz);
 

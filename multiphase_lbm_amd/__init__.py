@@ -149,6 +149,15 @@ class Context:
             raise ValueError("population array shape mismatch")
         self._check(self.lib.mplbm_write_populations_f64(self.h, _ptr(f)))
 
+    def save_checkpoint(self, path, step):
+        self._check(self.lib.mplbm_save_checkpoint(self.h, str(path).encode(), int(step)))
+
+    def load_checkpoint(self, path) -> int:
+        """Restore populations saved by save_checkpoint; returns the saved step."""
+        step = ctypes.c_int64()
+        self._check(self.lib.mplbm_load_checkpoint(self.h, str(path).encode(), ctypes.byref(step)))
+        return step.value
+
     def reduce(self) -> dict:
         s = _abi.Stats64()
         self._check(self.lib.mplbm_reduce(self.h, ctypes.byref(s)))

@@ -94,6 +94,8 @@ SIGNATURES = {
     "mplbm_read_fields_f64": (ctypes.c_int, [VP, VP, VP, VP]),
     "mplbm_read_populations_f64": (ctypes.c_int, [VP, VP]),
     "mplbm_write_populations_f64": (ctypes.c_int, [VP, VP]),
+    "mplbm_save_checkpoint": (ctypes.c_int, [VP, ctypes.c_char_p, ctypes.c_int64]),
+    "mplbm_load_checkpoint": (ctypes.c_int, [VP, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]),
     "mplbm_reduce": (ctypes.c_int, [VP, ctypes.POINTER(Stats64)]),
     "mplbm_set_solid_numbers": (ctypes.c_int, [VP, VP, ctypes.c_int64]),
     "mplbm_calc_cluster": (ctypes.c_int, [VP, ctypes.c_double, ctypes.c_double, ctypes.POINTER(Clusters), VP, VP]),

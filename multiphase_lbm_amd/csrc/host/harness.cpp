@@ -30,6 +30,7 @@
 #include <set>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "mplbm.h"
@@ -361,6 +362,77 @@ int write_vtk(const char* path, const Dims& d, const uint8_t* is_fluid, const fl
   return 0;
 }
 
+// Binary VTK XML RectilinearGrid with the point arrays of write_vtk (raw
+// appended data, UInt64 block sizes, little endian, x fastest).
+int write_vtr(const char* path, const Dims& d, const uint8_t* is_fluid, const float* rho, const float* pres,
+              const int64_t* liq, const int64_t* gas, const int64_t* solid) {
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return set_err("cannot open %s", path);
+  const size_t n = d.n();
+  struct Arr {
+    const char* type;
+    const char* name;
+    size_t count, esize;
+  };
+  const Arr arrs[] = {{"Int32", "Zones", n, 4},        {"Float32", "Density", n, 4}, {"Float32", "Pressure", n, 4},
+                      {"Int64", "LiqClusters", n, 8},  {"Int64", "GasClusters", n, 8}, {"Int64", "Solids", n, 8},
+                      {"Float32", "x", (size_t)d.lx, 4}, {"Float32", "y", (size_t)d.ly, 4},
+                      {"Float32", "z", (size_t)d.lz, 4}};
+  size_t off[9], o = 0;
+  for (int k = 0; k < 9; ++k) {
+    off[k] = o;
+    o += 8 + arrs[k].count * arrs[k].esize;
+  }
+  std::fprintf(f, "<?xml version=\"1.0\"?>\n");
+  std::fprintf(f, "<VTKFile type=\"RectilinearGrid\" version=\"1.0\" byte_order=\"LittleEndian\" "
+                  "header_type=\"UInt64\">\n");
+  std::fprintf(f, "  <RectilinearGrid WholeExtent=\"0 %ld 0 %ld 0 %ld\">\n", (long)d.lx - 1, (long)d.ly - 1,
+               (long)d.lz - 1);
+  std::fprintf(f, "    <Piece Extent=\"0 %ld 0 %ld 0 %ld\">\n", (long)d.lx - 1, (long)d.ly - 1, (long)d.lz - 1);
+  std::fprintf(f, "      <PointData Scalars=\"Density\">\n");
+  for (int k = 0; k < 6; ++k)
+    std::fprintf(f, "        <DataArray type=\"%s\" Name=\"%s\" format=\"appended\" offset=\"%zu\"/>\n",
+                 arrs[k].type, arrs[k].name, off[k]);
+  std::fprintf(f, "      </PointData>\n      <Coordinates>\n");
+  for (int k = 6; k < 9; ++k)
+    std::fprintf(f, "        <DataArray type=\"%s\" Name=\"%s\" format=\"appended\" offset=\"%zu\"/>\n",
+                 arrs[k].type, arrs[k].name, off[k]);
+  std::fprintf(f, "      </Coordinates>\n    </Piece>\n  </RectilinearGrid>\n");
+  std::fprintf(f, "  <AppendedData encoding=\"raw\">\n   _");
+  // point order x fastest, written in runs of lx values
+  std::vector<unsigned char> buf((size_t)d.lx * 8);
+  auto points = [&](size_t esize, auto value) {
+    const uint64_t bytes = (uint64_t)n * esize;
+    std::fwrite(&bytes, 8, 1, f);
+    for (int64_t z = 0; z < d.lz; ++z)
+      for (int64_t y = 0; y < d.ly; ++y) {
+        for (int64_t x = 0; x < d.lx; ++x) value(d.at(x, y, z), buf.data() + (size_t)x * esize);
+        std::fwrite(buf.data(), esize, (size_t)d.lx, f);
+      }
+  };
+  auto put32 = [](unsigned char* q, const void* v) { std::memcpy(q, v, 4); };
+  auto put64 = [](unsigned char* q, const void* v) { std::memcpy(q, v, 8); };
+  points(4, [&](size_t c, unsigned char* q) { const int32_t v = is_fluid[c] ? 0 : 1; put32(q, &v); });
+  points(4, [&](size_t c, unsigned char* q) { const float v = is_fluid[c] ? rho[c] : 0.f; put32(q, &v); });
+  points(4, [&](size_t c, unsigned char* q) { const float v = is_fluid[c] ? pres[c] : 0.f; put32(q, &v); });
+  points(8, [&](size_t c, unsigned char* q) { const int64_t v = liq ? liq[c] : 0; put64(q, &v); });
+  points(8, [&](size_t c, unsigned char* q) { const int64_t v = gas ? gas[c] : 0; put64(q, &v); });
+  points(8, [&](size_t c, unsigned char* q) { const int64_t v = solid ? solid[c] : 0; put64(q, &v); });
+  const float pas = (float)(1.0 / (double)d.lx);  // write_vtk's coordinates i/lx on every axis
+  for (int64_t L : {d.lx, d.ly, d.lz}) {
+    const uint64_t bytes = (uint64_t)L * 4;
+    std::fwrite(&bytes, 8, 1, f);
+    for (int64_t i = 0; i < L; ++i) {
+      const float v = (float)i * pas;
+      std::fwrite(&v, 4, 1, f);
+    }
+  }
+  std::fprintf(f, "\n  </AppendedData>\n</VTKFile>\n");
+  const bool bad = std::ferror(f) != 0;
+  std::fclose(f);
+  return bad ? set_err("write error on %s", path) : 0;
+}
+
 // ---------------------------------------------------------------------------
 // the MultiphaseLBM program (include/multiphase_lbm.h:26-218)
 
@@ -565,7 +637,18 @@ struct MultiphaseLBM {
       return true;
     };
     int rc = 0;
-    if (!check(mplbm_set_geometry(ctx, is_fluid.data())) || !check(mplbm_init_fields(ctx, rho.data()))) {
+    int64_t step = in.step;
+    if (!check(mplbm_set_geometry(ctx, is_fluid.data()))) {
+      mplbm_destroy(ctx);
+      return -1;
+    }
+    if (in.read_checkpoint) {  // exact restart (the reference restarts from densities only)
+      if (!check(mplbm_load_checkpoint(ctx, in.checkpoint_filename, &step))) {
+        mplbm_destroy(ctx);
+        return -1;
+      }
+      std::cout << "Resuming from checkpoint " << in.checkpoint_filename << " at step " << step << std::endl;
+    } else if (!check(mplbm_init_fields(ctx, rho.data()))) {
       mplbm_destroy(ctx);
       return -1;
     }
@@ -576,17 +659,29 @@ struct MultiphaseLBM {
     std::vector<int32_t> liq32, gas32;
     const int reverse = 1;
     const int64_t stat_period = in.statPeriod > 0 ? in.statPeriod : 1;
-    size_t vtk_num = 0;
+    const int64_t ck_period = in.checkpointPeriod;
+    size_t vtk_num = 0, vtr_num = 0;
     int64_t fluid_nodes = 0;
     for (uint8_t f : is_fluid) fluid_nodes += f;
+    // field files are written by a thread of their own while the GPU steps
+    // on; rho, pres and the label arrays are not touched until it is joined
+    std::thread writer;
+    int writer_rc = 0;
+    auto join_writer = [&]() {
+      if (writer.joinable()) writer.join();
+      if (writer_rc != 0) std::fprintf(stderr, "field file: %s\n", mplbm_host_last_error());
+      writer_rc = 0;
+    };
     const auto t0 = std::chrono::steady_clock::now();
-    int64_t step = in.step;
+    const int64_t step0 = step;
     while (step < in.step_max) {
-      // advance to the next statistics step in one call
+      // advance to the next statistics or checkpoint step in one call
       const int64_t next_stat = ((step + stat_period - 1) / stat_period) * stat_period;
-      const int64_t last = std::min(next_stat, in.step_max - 1);
+      int64_t last = std::min(next_stat, in.step_max - 1);
+      if (ck_period > 0) last = std::min(last, (step / ck_period + 1) * ck_period - 1);
       const int32_t n = (int32_t)(last - step + 1);
       const bool stat = (last % stat_period) == 0;
+      const bool ckpt = ck_period > 0 && (last + 1) % ck_period == 0;
       if (!check(mplbm_step(ctx, step, reverse, in.inject_type, n, stat ? MPLBM_STEP_WRITE_MACRO : 0))) {
         rc = -1;
         break;
@@ -594,21 +689,24 @@ struct MultiphaseLBM {
       step = last;
       if (stat) {
         const bool vtk = in.save_vtk_ascii && in.vtkPeriod > 0 && step % in.vtkPeriod == 0;
+        const bool vtr = in.save_vtr_binary && in.vtrPeriod > 0 && step % in.vtrPeriod == 0;
+        const bool files = vtk || vtr;
         mplbm_cluster_stats s{};
         if (in.host_statistics) {
+          join_writer();
           if (!check(mplbm_read_fields(ctx, rho.data(), pres.data(), nullptr))) { rc = -1; break; }
           cluster_stats(d, is_fluid.data(), rho.data(), pres.data(), solid_number.data(), (float)in.rhol_spinodal,
                         (float)in.rhog_spinodal, solid_count, liq_label.data(), gas_label.data(), &s);
         } else {
           // statistics on the device; fields and labels cross to the host
-          // only for a VTK file
-          if (vtk) {
+          // only for a field file
+          if (files) {
             liq32.resize(d.n());
             gas32.resize(d.n());
           }
           mplbm_clusters g{};
           if (!check(mplbm_calc_cluster(ctx, (float)in.rhol_spinodal, (float)in.rhog_spinodal, &g,
-                                        vtk ? liq32.data() : nullptr, vtk ? gas32.data() : nullptr))) {
+                                        files ? liq32.data() : nullptr, files ? gas32.data() : nullptr))) {
             rc = -1;
             break;
           }
@@ -621,7 +719,8 @@ struct MultiphaseLBM {
           s.ratio_liquid_vapor_boundary = g.ratio_liquid_vapor_boundary;
           s.cluster_radius = g.cluster_radius;
           s.fluid_count = g.fluid_count; s.liquid_count = g.liquid_count; s.vapor_count = g.vapor_count;
-          if (vtk) {
+          if (files) {
+            join_writer();
             if (!check(mplbm_read_fields(ctx, rho.data(), pres.data(), nullptr))) { rc = -1; break; }
             std::copy(liq32.begin(), liq32.end(), liq_label.begin());
             std::copy(gas32.begin(), gas32.end(), gas_label.begin());
@@ -630,21 +729,44 @@ struct MultiphaseLBM {
         std::printf("saturation=%f, aLp=%f,aVp=%f, step=%ld\n", s.saturation, s.avg_liq_press, s.avg_vap_press,
                     (long)step);
         write_stats_row(step, s);
-        if (vtk) {
-          std::cout << "Saving vtk ascii file number " << vtk_num << std::endl;
+        if (files) {
+          std::string vtk_name, vtr_name;
           char name[512];
-          std::snprintf(name, sizeof name, "%s/result%06zu.vtk", in.result_folder, vtk_num);
-          write_vtk(name, d, is_fluid.data(), rho.data(), pres.data(), liq_label.data(), gas_label.data(),
-                    solid_number.data());
-          ++vtk_num;
+          if (vtk) {
+            std::cout << "Saving vtk ascii file number " << vtk_num << std::endl;
+            std::snprintf(name, sizeof name, "%s/result%06zu.vtk", in.result_folder, vtk_num++);
+            vtk_name = name;
+          }
+          if (vtr) {
+            std::cout << "Saving vtr binary file number " << vtr_num << std::endl;
+            std::snprintf(name, sizeof name, "%s/result%06zu.vtr", in.result_folder, vtr_num++);
+            vtr_name = name;
+          }
+          join_writer();
+          writer = std::thread([this, vtk_name, vtr_name, &writer_rc]() {
+            int r = 0;
+            if (!vtk_name.empty())
+              r |= write_vtk(vtk_name.c_str(), d, is_fluid.data(), rho.data(), pres.data(), liq_label.data(),
+                             gas_label.data(), solid_number.data());
+            if (!vtr_name.empty())
+              r |= write_vtr(vtr_name.c_str(), d, is_fluid.data(), rho.data(), pres.data(), liq_label.data(),
+                             gas_label.data(), solid_number.data());
+            writer_rc = r;
+          });
         }
+      }
+      if (ckpt) {
+        const std::string name = std::string(in.result_folder) + "/checkpoint.mplbm";
+        if (!check(mplbm_save_checkpoint(ctx, name.c_str(), last + 1))) { rc = -1; break; }
+        std::cout << "Saved checkpoint at step " << last + 1 << std::endl;
       }
       ++step;
     }
+    join_writer();
     mplbm_sync(ctx);
     const double td = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::printf("GPU time for %ld steps on resolution of %ld: %f sec\n", (long)in.step_max, (long)d.lx, td);
-    const double upd = (double)fluid_nodes * (double)(in.step_max - in.step);
+    const double upd = (double)fluid_nodes * (double)(in.step_max - step0);
     std::printf("MFLUPS (incl. statistics): %.1f\n", td > 0 ? upd / td / 1e6 : 0.0);
     std::cout << std::endl;
     mplbm_destroy(ctx);
@@ -729,6 +851,12 @@ bool MultiphaseLBM::mplbm_parse_scalar_token(std::istream& file, const std::stri
     in.face_rho[2 * a + 1] = hi;
   } else if (token == "body_force") file >> in.body_force[0] >> in.body_force[1] >> in.body_force[2];
   else if (token == "host_statistics") file >> in.host_statistics;
+  else if (token == "checkpointPeriod") file >> in.checkpointPeriod;
+  else if (token == "read_checkpoint") {
+    file >> s;
+    std::snprintf(in.checkpoint_filename, sizeof in.checkpoint_filename, "%s", s.c_str());
+    in.read_checkpoint = 1;
+  }
   else return false;
   return true;
 }
@@ -915,8 +1043,12 @@ int64_t mplbm_host_generate_packing(uint8_t* is_fluid, int64_t lx, int64_t ly, i
             const float dx = px - xf, dy = py - yf, dz = pz - zf;
             if (dx * dx + dy * dy + dz * dz < rf * rf) {
               int ix = nearest(px), iy = nearest(py), iz = nearest(pz);
-              if (ix < 0) ix += (int)lx; if (iy < 0) iy += (int)ly; if (iz < 0) iz += (int)lz;
-              if (ix >= lx) ix -= (int)lx; if (iy >= ly) iy -= (int)ly; if (iz >= lz) iz -= (int)lz;
+              if (ix < 0) ix += (int)lx;
+              if (iy < 0) iy += (int)ly;
+              if (iz < 0) iz += (int)lz;
+              if (ix >= lx) ix -= (int)lx;
+              if (iy >= ly) iy -= (int)ly;
+              if (iz >= lz) iz -= (int)lz;
               if (ix < 0 || iy < 0 || iz < 0 || ix >= lx || iy >= ly || iz >= lz) continue;
               uint8_t& v = is_fluid[d.at(ix, iy, iz)];
               if (v) { v = 0; ++newly; }
@@ -949,6 +1081,12 @@ int mplbm_host_cluster_stats(const uint8_t* is_fluid, const float* rho, const fl
                              mplbm_cluster_stats* out) {
   return cluster_stats(Dims{lx, ly, lz}, is_fluid, rho, pres, solid_number, rhol, rhog, solid_count, liq_label,
                        gas_label, out);
+}
+
+int mplbm_host_write_vtr(const char* path, const uint8_t* is_fluid, const float* rho, const float* pres,
+                         const int64_t* liq, const int64_t* gas, const int64_t* solid, int64_t lx, int64_t ly,
+                         int64_t lz) {
+  return write_vtr(path, Dims{lx, ly, lz}, is_fluid, rho, pres, liq, gas, solid);
 }
 
 int mplbm_host_write_vtk(const char* path, const uint8_t* is_fluid, const float* rho, const float* pres,

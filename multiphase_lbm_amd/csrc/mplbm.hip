@@ -876,7 +876,6 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
 
   constexpr uint32_t sz = sizeof(S);
   const uint32_t psb = L.ps * sz;  // bytes of a population sub-plane
-  const uint32_t plb = kQ * psb;   // bytes of a plane
   const C psi_solid = psi_of<Mo>(ph, ph.rhos);
 
   // global coordinates of the rank region's rows and columns
@@ -1240,6 +1239,34 @@ __global__ void k_write_pops(Lattice<Mo> L, Phys<typename Mo::C> ph, const doubl
   for (int i = 0; i < kQ; ++i) compact_store_all(L, L.fout, i, x, r, encode<Mo>((C)in[(uint64_t)c * kQ + i], i, ph));
 }
 
+// Stored values (no decoding) of the 19 populations of every cell of plane
+// x, [y][z][19], solid cells 0: gather from fin, or scatter to fout (own
+// slots and the compact layout's ghost planes).
+template <class Mo>
+__global__ void k_plane_raw(Lattice<Mo> L, int x, typename Mo::S* buf, int scatter) {
+  using S = typename Mo::S;
+  const uint32_t r = blockIdx.x * 256u + threadIdx.x;
+  if (r >= L.plane) return;
+  const uint32_t c = (uint32_t)x * L.plane + r;
+  S* b = buf + (uint64_t)r * kQ;
+  const bool fluid = !(L.link[c] & 1u);
+  if (!scatter) {
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) b[i] = fluid ? L.fin[any_pix(L, i, c)] : S{};
+    return;
+  }
+  if (!fluid) return;
+  if (!L.compact) {
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) L.fout[L.pix(i, c)] = b[i];
+    return;
+  }
+  const int y = (int)(r / (uint32_t)L.lz), z = (int)(r - (uint32_t)y * L.lz);
+  const uint32_t rk = plane_rank(L.rw, L.W, L.ly, x, y, z);
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) compact_store_all(L, L.fout, i, x, rk, b[i]);
+}
+
 // ----------------------------------------------------------------------------
 // FP64 reductions: per-block partials, then one block folds them in order
 
@@ -1340,7 +1367,7 @@ struct mplbm_ctx {
   uint2* ctab = nullptr;        // [tile][x] list table
   size_t cpos_cap = 0, ctab_cap = 0;
   CTiling ctiling{};
-  std::vector<uint8_t> hmask;   // host copy of the solid mask (list rebuilds)
+  std::vector<uint8_t> hmask;   // host copy of the fluid mask (list rebuilds, checkpoints)
   size_t lat_bytes = 0;
   // cluster statistics (allocated at the first mplbm_calc_cluster)
   int32_t* cl_parent = nullptr;  // union-find parents, then cluster labels
@@ -1894,8 +1921,8 @@ mplbm_status mplbm_set_geometry(mplbm_ctx* ctx, const uint8_t* is_fluid) {
   ctx->fluid_nodes = nf;
   ctx->have_geometry = false;
   ctx->have_fields = false;
+  ctx->hmask.swap(m);
   if (ctx->compact) {
-    ctx->hmask.swap(m);
     const mplbm_status st = build_compact(ctx);
     if (st != MPLBM_OK) return st;
   }
@@ -2099,6 +2126,155 @@ mplbm_status mplbm_write_populations_f64(mplbm_ctx* ctx, const double* f) {
   (void)hipFree(d);
   if (e != hipSuccess || e2 != hipSuccess)
     return fail(ctx, MPLBM_ERR_HIP, "write populations: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+  return MPLBM_OK;
+}
+
+}  // extern "C"
+
+// Checkpoint file: header, then plane by plane the stored values of the 19
+// populations of each fluid cell in [y][z] order.
+struct CheckpointHeader {
+  char magic[8];          // "MPLBMCK1"
+  uint32_t version;       // 1
+  uint32_t storage;       // mplbm_storage
+  int64_t lx, ly, lz;
+  int64_t step;
+  int64_t fluid_nodes;
+  uint64_t mask_hash;     // FNV-1a of the fluid mask
+  double rho_ref, dev_scale;
+  uint32_t bytes_per_pop;
+  uint32_t reserved;
+};
+
+static uint64_t mask_hash(const std::vector<uint8_t>& m) {
+  uint64_t h = 1469598103934665603ull;
+  for (uint8_t v : m) h = (h ^ v) * 1099511628211ull;
+  return h;
+}
+
+struct FileCloser {
+  FILE* f;
+  ~FileCloser() {
+    if (f) fclose(f);
+  }
+};
+
+template <class Fn>
+static mplbm_status plane_raw_pass(mplbm_ctx* ctx, bool scatter, Fn&& per_plane) {
+  const uint64_t plane = ctx->cfg.ly * ctx->cfg.lz;
+  const size_t bytes = plane * kQ * (size_t)ctx->bytes_pop;
+  void* d = nullptr;
+  HIP_TRY(ctx, hipMalloc(&d, bytes));
+  std::vector<unsigned char> h(bytes);
+  mplbm_status st = MPLBM_OK;
+  for (int64_t x = 0; x < ctx->cfg.lx && st == MPLBM_OK; ++x) {
+    if (scatter) {
+      st = per_plane(x, h.data());
+      if (st != MPLBM_OK) break;
+      if (hipMemcpyAsync(d, h.data(), bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
+        st = fail(ctx, MPLBM_ERR_HIP, "checkpoint upload");
+        break;
+      }
+    }
+    st = dispatch(ctx, [&](auto mo) -> mplbm_status {
+      using Mo = decltype(mo);
+      auto L = make_lattice<Mo>(ctx, ctx->cur, ctx->cur);
+      hipLaunchKernelGGL(k_plane_raw<Mo>, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, ctx->stream, L,
+                         (int)x, reinterpret_cast<typename Mo::S*>(d), scatter ? 1 : 0);
+      return MPLBM_OK;
+    });
+    if (st != MPLBM_OK) break;
+    if (hipGetLastError() != hipSuccess) {
+      st = fail(ctx, MPLBM_ERR_HIP, "checkpoint kernel");
+      break;
+    }
+    if (!scatter) {
+      if (hipMemcpyAsync(h.data(), d, bytes, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+          hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        st = fail(ctx, MPLBM_ERR_HIP, "checkpoint download");
+        break;
+      }
+      st = per_plane(x, h.data());
+    } else if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+      st = fail(ctx, MPLBM_ERR_HIP, "checkpoint scatter");
+    }
+  }
+  (void)hipFree(d);
+  return st;
+}
+
+extern "C" {
+
+mplbm_status mplbm_save_checkpoint(mplbm_ctx* ctx, const char* path, int64_t step) {
+  if (!ctx || !path) return fail(ctx, MPLBM_ERR_ARG, "null argument");
+  if (!ctx->have_fields) return fail(ctx, MPLBM_ERR_STATE, "fields not initialised");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  FileCloser fc{fopen(path, "wb")};
+  if (!fc.f) return fail(ctx, MPLBM_ERR_ARG, "cannot open %s for writing", path);
+  CheckpointHeader hd{};
+  memcpy(hd.magic, "MPLBMCK1", 8);
+  hd.version = 1;
+  hd.storage = (uint32_t)ctx->cfg.storage;
+  hd.lx = ctx->cfg.lx;
+  hd.ly = ctx->cfg.ly;
+  hd.lz = ctx->cfg.lz;
+  hd.step = step;
+  hd.fluid_nodes = ctx->fluid_nodes;
+  hd.mask_hash = mask_hash(ctx->hmask);
+  hd.rho_ref = ctx->rho_ref;
+  hd.dev_scale = ctx->scale;
+  hd.bytes_per_pop = (uint32_t)ctx->bytes_pop;
+  if (fwrite(&hd, sizeof hd, 1, fc.f) != 1) return fail(ctx, MPLBM_ERR_ARG, "write error on %s", path);
+  const uint64_t plane = ctx->cfg.ly * ctx->cfg.lz;
+  const size_t rec = kQ * (size_t)ctx->bytes_pop;
+  return plane_raw_pass(ctx, false, [&](int64_t x, unsigned char* h) -> mplbm_status {
+    const uint8_t* m = ctx->hmask.data() + (size_t)x * plane;
+    for (uint64_t r = 0; r < plane; ++r)
+      if (m[r] && fwrite(h + r * rec, rec, 1, fc.f) != 1) return fail(ctx, MPLBM_ERR_ARG, "write error on %s", path);
+    return MPLBM_OK;
+  });
+}
+
+mplbm_status mplbm_load_checkpoint(mplbm_ctx* ctx, const char* path, int64_t* step) {
+  if (!ctx || !path) return fail(ctx, MPLBM_ERR_ARG, "null argument");
+  if (!ctx->have_geometry) return fail(ctx, MPLBM_ERR_STATE, "geometry not set");
+  FileCloser fc{fopen(path, "rb")};
+  if (!fc.f) return fail(ctx, MPLBM_ERR_ARG, "cannot open %s", path);
+  CheckpointHeader hd{};
+  if (fread(&hd, sizeof hd, 1, fc.f) != 1 || memcmp(hd.magic, "MPLBMCK1", 8) != 0 || hd.version != 1)
+    return fail(ctx, MPLBM_ERR_ARG, "%s is not a checkpoint of this library", path);
+  if (hd.lx != ctx->cfg.lx || hd.ly != ctx->cfg.ly || hd.lz != ctx->cfg.lz)
+    return fail(ctx, MPLBM_ERR_ARG, "checkpoint domain %lldx%lldx%lld differs from the context's",
+                (long long)hd.lx, (long long)hd.ly, (long long)hd.lz);
+  if ((int)hd.storage != ctx->cfg.storage || (int)hd.bytes_per_pop != ctx->bytes_pop)
+    return fail(ctx, MPLBM_ERR_ARG, "checkpoint storage %u differs from the context's", hd.storage);
+  if (hd.fluid_nodes != ctx->fluid_nodes || hd.mask_hash != mask_hash(ctx->hmask))
+    return fail(ctx, MPLBM_ERR_ARG, "checkpoint geometry differs from the context's");
+  // the stored deviations are relative to the file's reference density and scale
+  ctx->rho_ref = (float)hd.rho_ref;
+  ctx->scale = (float)hd.dev_scale;
+  const uint64_t plane = ctx->cfg.ly * ctx->cfg.lz;
+  const size_t rec = kQ * (size_t)ctx->bytes_pop;
+  mplbm_status st = plane_raw_pass(ctx, true, [&](int64_t x, unsigned char* h) -> mplbm_status {
+    const uint8_t* m = ctx->hmask.data() + (size_t)x * plane;
+    for (uint64_t r = 0; r < plane; ++r)
+      if (m[r] && fread(h + r * rec, rec, 1, fc.f) != 1) return fail(ctx, MPLBM_ERR_ARG, "%s is truncated", path);
+    return MPLBM_OK;
+  });
+  if (st != MPLBM_OK) return st;
+  if (!ctx->compact) {  // bounce-back slots of the full layout
+    st = dispatch(ctx, [&](auto mo) -> mplbm_status {
+      using Mo = decltype(mo);
+      auto L = make_lattice<Mo>(ctx, ctx->cur, ctx->cur);
+      hipLaunchKernelGGL(k_ghost<Mo>, dim3((unsigned)((ctx->ncell + 255) / 256)), dim3(256), 0, ctx->stream, L);
+      return MPLBM_OK;
+    });
+    if (st != MPLBM_OK) return st;
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  ctx->have_fields = true;
+  if (step) *step = hd.step;
   return MPLBM_OK;
 }
 

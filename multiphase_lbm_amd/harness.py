@@ -38,7 +38,8 @@ class Input(ctypes.Structure):
         ("storage", ctypes.c_int32), ("kernel", ctypes.c_int32), ("math", ctypes.c_int32),
         ("face_type", ctypes.c_int32 * 6), ("face_rho", ctypes.c_double * 6),
         ("body_force", ctypes.c_double * 3), ("unknown_tokens", ctypes.c_int32),
-        ("host_statistics", ctypes.c_int32),
+        ("host_statistics", ctypes.c_int32), ("read_checkpoint", ctypes.c_int32),
+        ("checkpointPeriod", I64), ("checkpoint_filename", ctypes.c_char * 256),
     ]
 
     def as_dict(self):
@@ -81,6 +82,7 @@ SIGNATURES = {
     "mplbm_host_cluster_stats": (ctypes.c_int, [VP, VP, VP, VP, I64, I64, I64, ctypes.c_double, ctypes.c_double,
                                                 I64, VP, VP, ctypes.POINTER(ClusterStats)]),
     "mplbm_host_write_vtk": (ctypes.c_int, [ctypes.c_char_p, VP, VP, VP, VP, VP, VP, I64, I64, I64]),
+    "mplbm_host_write_vtr": (ctypes.c_int, [ctypes.c_char_p, VP, VP, VP, VP, VP, VP, I64, I64, I64]),
     "mplbm_host_run": (ctypes.c_int, [ctypes.c_char_p]),
 }
 
@@ -189,6 +191,40 @@ def cluster_stats(is_fluid, rho, pres, solid_number=None, rhol=0.2726, rhog=0.04
     out = s.as_dict()
     if want_labels:
         return out, liq, gas
+    return out
+
+
+def write_vtr(path, is_fluid, rho, pres, liq=None, gas=None, solid_number=None):
+    shape = is_fluid.shape
+    m = np.ascontiguousarray(is_fluid, np.uint8)
+    r = np.ascontiguousarray(rho, np.float32)
+    p = np.ascontiguousarray(pres, np.float32)
+    li = None if liq is None else np.ascontiguousarray(liq, np.int64)
+    ga = None if gas is None else np.ascontiguousarray(gas, np.int64)
+    sn = None if solid_number is None else np.ascontiguousarray(solid_number, np.int64)
+    if load().mplbm_host_write_vtr(str(path).encode(), _p(m), _p(r), _p(p), _p(li), _p(ga), _p(sn), *shape) != 0:
+        raise _err("write_vtr")
+
+
+def read_vtr(path):
+    """Arrays of a file written by write_vtr, as [x][y][z] numpy arrays, and
+    the coordinates."""
+    import re
+    data = open(path, "rb").read()
+    head, _, rest = data.partition(b"<AppendedData encoding=\"raw\">")
+    base = rest.index(b"_") + 1
+    ext = [int(v) for v in re.search(rb'WholeExtent="([^"]+)"', head).group(1).split()]
+    shape = (ext[1] + 1, ext[3] + 1, ext[5] + 1)
+    out = {}
+    for m in re.finditer(rb'<DataArray type="(\w+)" Name="(\w+)" format="appended" offset="(\d+)"', head):
+        typ, name, off = m.group(1).decode(), m.group(2).decode(), int(m.group(3))
+        n = int(np.frombuffer(rest, np.uint64, 1, base + off)[0])
+        dt = {"Float32": np.float32, "Int64": np.int64, "Int32": np.int32}[typ]
+        a = np.frombuffer(rest, dt, n // np.dtype(dt).itemsize, base + off + 8)
+        if name in ("x", "y", "z"):
+            out[name] = a.copy()
+        else:
+            out[name] = a.reshape(shape[::-1]).transpose(2, 1, 0).copy()  # x fastest in the file
     return out
 
 

@@ -5,6 +5,7 @@ field read (the reference's way, input token host_statistics 1) must write
 the same Stats.txt rows and VTK files."""
 import os
 
+import numpy as np
 import pytest
 
 from multiphase_lbm_amd import harness
@@ -29,18 +30,19 @@ place_sphere 9 8 7 5
 place_sphere 27 20 21 6
 place_sphere 30 5 12 4
 place_sphere 3 25 25 4.5
-step_max 241
+step_max {step_max}
 kernel {kernel}
 host_statistics {host}
+{extra}
 """
 
 
-def _run(tmp_path, kernel, host):
-    folder = tmp_path / f"{kernel}_{host}"
+def _run(tmp_path, kernel, host, step_max=241, extra="", tag=""):
+    folder = tmp_path / f"{kernel}_{host}{tag}"
     folder.mkdir()
-    inp = tmp_path / f"in_{kernel}_{host}.txt"
-    inp.write_text(INPUT.format(folder=folder, kernel=kernel, host=host))
-    print(f"case driver kernel={kernel} host_statistics={host}", flush=True)
+    inp = tmp_path / f"in_{kernel}_{host}{tag}.txt"
+    inp.write_text(INPUT.format(folder=folder, kernel=kernel, host=host, step_max=step_max, extra=extra))
+    print(f"case driver kernel={kernel} host_statistics={host} {extra!r}", flush=True)
     assert harness.load().mplbm_host_run(str(inp).encode()) == 0, harness.load().mplbm_host_last_error()
     return folder
 
@@ -62,3 +64,40 @@ def test_device_and_host_statistics_agree(tmp_path, kernel):
     assert vtks == sorted(f for f in os.listdir(host) if f.endswith(".vtk")) and len(vtks) == 4
     for f in vtks:
         assert (dev / f).read_bytes() == (host / f).read_bytes(), f
+
+
+def _vtk_arrays(path, shape):
+    """Point arrays of an ASCII VTK written by write_vtk, as [x][y][z]."""
+    lines = path.read_text().splitlines()
+    n = shape[0] * shape[1] * shape[2]
+    out = {}
+    for k, ln in enumerate(lines):
+        if ln.startswith("SCALARS "):
+            name = ln.split()[1]
+            vals = np.array([float(v) for v in lines[k + 2:k + 2 + n]])
+            out[name] = vals.reshape(shape[::-1]).transpose(2, 1, 0)
+    return out
+
+
+def test_checkpoint_resume_and_binary_vtr(tmp_path):
+    # uninterrupted run with VTR files next to the VTK files
+    full = _run(tmp_path, "compact", 0, extra="vtrPeriod 80", tag="_full")
+    # the same run saved at step 120, then resumed from the checkpoint
+    first = _run(tmp_path, "compact", 0, step_max=121, extra="checkpointPeriod 120", tag="_a")
+    ck = first / "checkpoint.mplbm"
+    assert ck.exists()
+    resumed = _run(tmp_path, "compact", 0, extra=f"read_checkpoint {ck}", tag="_b")
+    rows_full = {r[0]: r for r in _rows(full)}
+    rows_res = _rows(resumed)
+    assert [r[0] for r in rows_res] == ["120", "160", "200", "240"]
+    for r in rows_res:
+        assert r == rows_full[r[0]], (r, rows_full[r[0]])
+    # VTR holds the VTK arrays (the VTK rounds floats to 5 digits)
+    shape = (36, 30, 28)
+    for k in range(4):
+        v = harness.read_vtr(full / f"result{k:06d}.vtr")
+        a = _vtk_arrays(full / f"result{k:06d}.vtk", shape)
+        for name in ("Zones", "LiqClusters", "GasClusters", "Solids"):
+            assert np.array_equal(v[name], a[name].astype(np.int64)), name
+        for name in ("Density", "Pressure"):
+            assert np.allclose(v[name], a[name], rtol=6e-5, atol=1e-30), name

@@ -43,13 +43,16 @@ def test_parse_input_inject_txt():
 def test_parse_extension_tokens(tmp_path):
     p = tmp_path / "in.txt"
     p.write_text("lx 32 ly 16 lz 8 # sizes\nstorage fp16s\nkernel twopass\npressure_x 2.0 0.15\n"
-                 "body_force 1e-6 0 0\nbogus_token\n")
+                 "body_force 1e-6 0 0\nbogus_token\ncheckpointPeriod 500\nread_checkpoint run/ck.mplbm\n"
+                 "host_statistics 1\n")
     d = H.parse_input(p)
     assert (d["lx"], d["ly"], d["lz"]) == (32, 16, 8)
     assert d["storage"] == 2 and d["kernel"] == 2
     assert d["face_type"][:2] == [1, 1] and d["face_rho"][:2] == [2.0, 0.15]
     assert d["body_force"] == [1e-6, 0.0, 0.0]
     assert d["unknown_tokens"] == 1
+    assert d["checkpointPeriod"] == 500 and d["read_checkpoint"] == 1
+    assert d["checkpoint_filename"] == "run/ck.mplbm" and d["host_statistics"] == 1
 
 
 def test_place_sphere_integer_centre_is_the_open_ball():
@@ -150,6 +153,36 @@ def test_vtk_roundtrip(tmp_path):
     assert n == 60
     want = np.where(fluid == 1, np.array([float(f"{v:.4e}") for v in rho.ravel()]).reshape(shape), 0.0)
     assert np.array_equal(back, want.astype(np.float32))
+
+
+def test_vtr_binary_roundtrip(tmp_path):
+    """The binary VTR file holds exactly the arrays, in x-fastest point
+    order, with write_vtk's coordinates; its XML is well formed."""
+    import xml.etree.ElementTree as ET
+    shape = (5, 4, 3)
+    rng = np.random.default_rng(1)
+    fluid = (rng.uniform(size=shape) > 0.2).astype(np.uint8)
+    rho = rng.uniform(0.1, 2.0, shape).astype(np.float32)
+    p = rng.uniform(0.0, 0.1, shape).astype(np.float32)
+    liq = rng.integers(0, 5, shape).astype(np.int64)
+    gas = rng.integers(0, 3, shape).astype(np.int64)
+    sn = np.where(fluid == 0, rng.integers(1, 9, shape), 0).astype(np.int64)
+    path = tmp_path / "result000000.vtr"
+    H.write_vtr(path, fluid, rho, p, liq, gas, sn)
+    d = H.read_vtr(path)
+    assert np.array_equal(d["Density"], np.where(fluid == 1, rho, 0).astype(np.float32))
+    assert np.array_equal(d["Pressure"], np.where(fluid == 1, p, 0).astype(np.float32))
+    assert np.array_equal(d["Zones"], (fluid == 0).astype(np.int32))
+    assert np.array_equal(d["LiqClusters"], liq) and np.array_equal(d["GasClusters"], gas)
+    assert np.array_equal(d["Solids"], sn)
+    pas = np.float32(1.0 / shape[0])
+    assert np.array_equal(d["x"], np.arange(5, dtype=np.float32) * pas)
+    assert np.array_equal(d["z"], np.arange(3, dtype=np.float32) * pas)
+    raw = path.read_bytes()
+    head = raw[:raw.index(b"<AppendedData")] + b"</VTKFile>"
+    root = ET.fromstring(head.replace(b"</VTKFile>", b"") + b"</VTKFile>")
+    assert root.attrib["type"] == "RectilinearGrid" and root.attrib["header_type"] == "UInt64"
+    assert raw.rstrip().endswith(b"</VTKFile>")
 
 
 def test_cluster_stats_hand_built():
