@@ -657,7 +657,12 @@ struct MultiphaseLBM {
       return -1;
     }
     std::vector<int32_t> liq32, gas32;
-    const int reverse = 1;
+    // reverse = 1 injects, -1 drains (src/multiphase_lbm.cu:571-573).  With
+    // max_saturation / min_saturation in the input (the reference parses
+    // them, :712-716, but never changes reverse) the driver cycles: inject
+    // until the saturation reaches max_saturation, then drain until it falls
+    // to min_saturation, and so on, switching at statistics steps.
+    int reverse = 1;
     const int64_t stat_period = in.statPeriod > 0 ? in.statPeriod : 1;
     const int64_t ck_period = in.checkpointPeriod;
     size_t vtk_num = 0, vtr_num = 0;
@@ -729,6 +734,15 @@ struct MultiphaseLBM {
         std::printf("saturation=%f, aLp=%f,aVp=%f, step=%ld\n", s.saturation, s.avg_liq_press, s.avg_vap_press,
                     (long)step);
         write_stats_row(step, s);
+        if (in.hysteresis) {
+          if (reverse == 1 && s.saturation >= in.max_saturation) {
+            reverse = -1;
+            std::printf("saturation %f reached max_saturation at step %ld: draining\n", s.saturation, (long)step);
+          } else if (reverse == -1 && s.saturation <= in.min_saturation) {
+            reverse = 1;
+            std::printf("saturation %f reached min_saturation at step %ld: injecting\n", s.saturation, (long)step);
+          }
+        }
         if (files) {
           std::string vtk_name, vtr_name;
           char name[512];

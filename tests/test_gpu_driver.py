@@ -101,3 +101,47 @@ def test_checkpoint_resume_and_binary_vtr(tmp_path):
             assert np.array_equal(v[name], a[name].astype(np.int64)), name
         for name in ("Density", "Pressure"):
             assert np.allclose(v[name], a[name], rtol=6e-5, atol=1e-30), name
+
+
+HYSTERESIS = """result_folder {folder}
+statPeriod 25
+lx 32
+ly 32
+lz 32
+rho0 0.7
+IniPerturbRate 0.0285714
+rhos 0.7
+G -5.0
+tau 1.0
+rhol_spinodal 1.285931
+rhog_spinodal 0.323507
+resize
+place_sphere 16 16 16 6
+rho_inject_period 5
+rho_increment 0.004
+inject_type 0
+inject_position 1
+max_saturation 0.5
+min_saturation 0.3
+step_max 3001
+kernel compact
+"""
+
+
+def test_hysteresis_cycles_injection_and_drainage(tmp_path):
+    """max_saturation / min_saturation (parsed, never acted on in the
+    reference): the driver injects until the saturation reaches the maximum,
+    drains to the minimum, then injects again."""
+    folder = tmp_path / "hyst"
+    folder.mkdir()
+    inp = tmp_path / "hyst.txt"
+    inp.write_text(HYSTERESIS.format(folder=folder))
+    print("case driver hysteresis", flush=True)
+    assert harness.load().mplbm_host_run(str(inp).encode()) == 0, harness.load().mplbm_host_last_error()
+    sat = np.array([float(r[8]) for r in _rows(folder)])
+    assert np.isfinite(sat).all()
+    i = int(np.argmax(sat >= 0.5))
+    assert sat[i] >= 0.5, "never reached max_saturation"
+    j = i + int(np.argmax(sat[i:] <= 0.3))
+    assert sat[j] <= 0.3, "never drained to min_saturation"
+    assert sat[j:].max() > sat[j] + 0.05, "injection did not resume"
