@@ -19,7 +19,7 @@ extern "C" {
 #endif
 
 /* Parsed input file (tokens of src/multiphase_lbm.cu:631-725 plus the
- * runtime-size and boundary tokens of this build). */
+ * runtime-size, storage, kernel, boundary, statistics and checkpoint tokens). */
 typedef struct mplbm_input {
   int64_t lx, ly, lz;
   char result_folder[256];

@@ -1,6 +1,6 @@
 """Pins of the oracle against facts fixed by the mathematics of the method
-(the reference ships no tests or golden vectors, SURVEY of the reference:
-no test directory; its kernels need nvcc, which this image lacks).
+(the reference ships no tests or golden vectors: it has no test directory,
+and its kernels need nvcc, which is not available here).
 
 Each test is chosen so that a plausible slip in the oracle (a wrong sign or
 index in a velocity table, a dropped force term, a transposed neighbour, a
