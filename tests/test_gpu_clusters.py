@@ -69,7 +69,9 @@ def _blobs(shape, seed, nblobs=12, ndrops=25):
 def _compare(ctx, fluid, sn, nsolid):
     rho, p, _ = ctx.read_fields(np.float32)
     got, gl, gg = ctx.calc_cluster(RHOL, RHOG, want_labels=True)
-    ref, rl, rg = harness.cluster_stats(fluid, rho, p, sn, RHOL, RHOG, nsolid, want_labels=True)
+    # thresholds as the reference's floats (the driver passes them so too)
+    ref, rl, rg = harness.cluster_stats(fluid, rho, p, sn, float(np.float32(RHOL)), float(np.float32(RHOG)), nsolid,
+                                        want_labels=True)
     assert np.array_equal(gl.astype(np.int64), rl), "LiqClusters labels differ"
     assert np.array_equal(gg.astype(np.int64), rg), "GasClusters labels differ"
     for k in EXACT:
@@ -129,6 +131,23 @@ def test_after_two_phase_steps():
         ctx.step(0, 400, write_macro=True)
         got = _compare(ctx, fluid, sn, nsolid)
     assert got["liquid_count"] > 0 and got["vapor_count"] > 0
+
+
+def test_many_clusters_after_short_run():
+    # early spinodal decomposition: thousands of small clusters, many
+    # interface nodes near the thresholds
+    shape = (96, 96, 64)
+    print(f"case clusters many shape={shape}", flush=True)
+    fluid, sn, nsolid = _packing(shape, 40, 3.0, 7.0, seed=8)
+    rng = np.random.default_rng(6)
+    rho = (0.7 * (1 + 0.02 * (rng.random(shape) - 0.5))).astype(np.float32)
+    with mp.Context(*shape, storage="fp32", kernel="compact", G=-5.0, tau=1.0, rhos=0.7) as ctx:
+        ctx.set_geometry(fluid)
+        ctx.set_solid_numbers(sn, nsolid)
+        ctx.init_fields(rho)
+        ctx.step(0, 200, write_macro=True)
+        got = _compare(ctx, fluid, sn, nsolid)
+    assert got["liq_cluster_count"] > 50
 
 
 def test_large_domain_and_no_solid_numbers():
