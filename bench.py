@@ -42,7 +42,8 @@ def parse():
     ap.add_argument("--rmin", type=float, default=6.0)
     ap.add_argument("--rmax", type=float, default=18.0)
     ap.add_argument("--seed", type=int, default=3)
-    ap.add_argument("--storage", default="fp32,fp32fast,fp16s,bf16s")
+    ap.add_argument("--storage", default="fp32,fp32fast,fp16s,bf16s,fp16sr",
+                    help="fp32, fp32fast, fp16s, bf16s, fp64; fp16sr/bf16sr: 16-bit with stochastic rounding")
     ap.add_argument("--kernel", default="compact", help="compact (fluid nodes only), fused or twopass")
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--tile-z", type=int, default=0)
@@ -86,6 +87,8 @@ def workload(a):
 
 
 PHYS = dict(tau=1.0, G=-5.0, rhos=0.7, rhol_spinodal=1.285931, rhog_spinodal=0.323507)
+# storage names of bench.py and the stored form they time
+BASE_STORAGE = {"fp32fast": "fp32", "fp16sr": "fp16s", "bf16sr": "bf16s"}
 
 
 def run_gpu(a, shape, fluid, rho, storage):
@@ -113,7 +116,7 @@ def run_gpu(a, shape, fluid, rho, storage):
     nf = info["fluid_nodes"]
     launches = a.steps * (2 if a.kernel == "twopass" else 1)
     per_launch_s = ev_ms / 1e3 / launches
-    bpop = mp.STORAGE_BYTES[mp.STORAGE_NAMES[storage.replace("fast", "")]]
+    bpop = mp.STORAGE_BYTES[mp.STORAGE_NAMES[BASE_STORAGE.get(storage, storage)]]
     alg_bytes = 2 * Q * bpop * nf  # per step
     return {
         "mlups": nf * a.steps / wall / 1e6,
@@ -191,10 +194,15 @@ def precision_study(a, shape, fluid, rho):
     sat_thr = 0.5 * (1.932442 + 0.156413)
     two = {"steps": steps, "fp64": {"saturation": float((ref_r[fl] > sat_thr).mean()),
                                     "mass_rel_drift": ref_st["mass"] / float(rho.astype(np.float64)[fl].sum()) - 1}}
-    variants = [("fp32", {}), ("fp16s", {}), ("fp16s", {"rho_ref": 1.04}), ("bf16s", {}), ("bf16s", {"rho_ref": 1.04})]
+    variants = [("fp32", {}), ("fp16s", {}), ("fp16s", {"rounding": "stochastic"}), ("bf16s", {}),
+                ("bf16s", {"rounding": "stochastic"})]
+
+    def vkey(storage, kw):
+        return storage + ("_stochastic" if kw.get("rounding") == "stochastic" else "")
+
     for storage, kw in variants:
         r, _, st, dt, info = run(storage, PHYS, steps, rho, **kw)
-        key = storage + ("" if not kw else f"_rhoref{kw['rho_ref']}")
+        key = vkey(storage, kw)
         two[key] = {
             "density_rel_l2": float(np.linalg.norm(r[fl] - ref_r[fl]) / np.linalg.norm(ref_r[fl])),
             "density_max_abs": float(np.abs(r[fl] - ref_r[fl]).max()),
@@ -225,7 +233,7 @@ def precision_study(a, shape, fluid, rho):
 
     sl = {"steps": a.slab_steps, "fp64": run_slab("fp64")}
     for storage, kw in variants:
-        key = storage + ("" if not kw else f"_rhoref{kw['rho_ref']}")
+        key = vkey(storage, kw)
         v = run_slab(storage, **kw)
         v["rho_l_rel_err"] = abs(v["rho_l"] - sl["fp64"]["rho_l"]) / sl["fp64"]["rho_l"]
         v["rho_v_rel_err"] = abs(v["rho_v"] - sl["fp64"]["rho_v"]) / sl["fp64"]["rho_v"]
@@ -238,9 +246,10 @@ def precision_study(a, shape, fluid, rho):
     nu = (tau - 0.5) / 3
     perm = {"steps": a.perm_steps, "tau": tau, "g": g}
     ones = np.ones(shape)
-    for storage in ("fp64", "fp32", "fp16s", "bf16s"):
+    for storage, kw in [("fp64", {})] + variants:
         r, u, st, dt, info = run(storage, dict(tau=tau, G=0.0, rhos=1.0, body_force=(g, 0.0, 0.0)), a.perm_steps,
-                                 ones.astype(np.float64 if storage == "fp64" else np.float32))
+                                 ones.astype(np.float64 if storage == "fp64" else np.float32), **kw)
+        storage = vkey(storage, kw)
         # physical velocity (m + F/2)/rho = shifted u - (tau - 1/2) g; superficial average over all cells
         ux = np.where(fl, u[..., 0] - (tau - 0.5) * g, 0.0)
         k = nu * float(ux.mean()) / g

@@ -84,6 +84,14 @@ typedef enum mplbm_bounce_back { MPLBM_BB_WRITE = 0, MPLBM_BB_PULL = 1 } mplbm_b
  * rule of include/iteration_device.cuh:213-243. */
 typedef enum mplbm_face { MPLBM_FACE_PERIODIC = 0, MPLBM_FACE_PRESSURE = 1 } mplbm_face;
 
+/* Rounding of the 16-bit deviations stored by each step (FP16S/BF16S with
+ * the COMPACT kernel).  NEAREST: round to nearest even, the same rounding
+ * every step, so a steady population keeps the same rounding error and mass
+ * drifts linearly.  STOCHASTIC: a deterministic per-(step, node, population)
+ * dither of one unit in the last place before rounding to nearest, which
+ * makes the stored value unbiased; runs stay reproducible. */
+typedef enum mplbm_rounding { MPLBM_ROUND_NEAREST = 0, MPLBM_ROUND_STOCHASTIC = 1 } mplbm_rounding;
+
 /* All fields of transfer_parameter (src/iteration_host.cu:16-41) plus the
  * domain size, storage, faces, body force and kernel tiling.  Fill with
  * mplbm_config_default() and override. */
@@ -113,8 +121,9 @@ typedef struct mplbm_config {
   int32_t seg_x;               /* FUSED, COMPACT: planes marched per block (0 = tuned default) */
   int32_t plane_pad;           /* extra elements per population sub-plane (-1 = automatic) */
   int32_t bounce_back;         /* mplbm_bounce_back: how FUSED resolves halfway bounce-back */
-  int32_t block_threads;       /* COMPACT: workgroup size, 256 or 512 (0 = tuned default); interior
-                                  nodes of a plane beyond it are pulled a second time */
+  int32_t block_threads;       /* COMPACT: workgroup size 256, 512 or 1024 (0 = tuned default), at
+                                  least the tile's interior node count */
+  int32_t rounding;            /* mplbm_rounding of the 16-bit forms */
 } mplbm_config;
 
 /* FP64 reductions over fluid nodes (mplbm_reduce). */

@@ -13,10 +13,12 @@ import numpy as np
 
 from . import _abi
 from ._abi import (BF16S, FACE_PERIODIC, FACE_PRESSURE, FP16S, FP32, FP64, KERNEL_AUTO, KERNEL_COMPACT,
-                   KERNEL_FUSED, KERNEL_TWOPASS, STEP_WRITE_MACRO, STORAGE_BYTES, STORAGE_NAMES)
+                   KERNEL_FUSED, KERNEL_TWOPASS, ROUND_NEAREST, ROUND_STOCHASTIC, STEP_WRITE_MACRO, STORAGE_BYTES,
+                   STORAGE_NAMES)
 
 __all__ = ["Context", "MplbmError", "FP32", "FP64", "FP16S", "BF16S", "KERNEL_AUTO", "KERNEL_FUSED",
-           "KERNEL_TWOPASS", "KERNEL_COMPACT", "FACE_PERIODIC", "FACE_PRESSURE", "default_config"]
+           "KERNEL_TWOPASS", "KERNEL_COMPACT", "FACE_PERIODIC", "FACE_PRESSURE", "ROUND_NEAREST",
+           "ROUND_STOCHASTIC", "default_config"]
 
 KERNEL_NAMES = {"auto": KERNEL_AUTO, "fused": KERNEL_FUSED, "twopass": KERNEL_TWOPASS, "compact": KERNEL_COMPACT}
 
@@ -52,6 +54,8 @@ class Context:
         cfg.lx, cfg.ly, cfg.lz = int(lx), int(ly), int(lz)
         if storage == "fp32fast":
             storage, params = "fp32", dict(params, math="fast")
+        elif storage in ("fp16sr", "bf16sr"):
+            storage, params = storage[:-1], dict(params, rounding="stochastic")
         cfg.storage = STORAGE_NAMES[storage] if isinstance(storage, str) else int(storage)
         cfg.kernel = KERNEL_NAMES[kernel] if isinstance(kernel, str) else int(kernel)
         for k, v in params.items():
@@ -63,6 +67,8 @@ class Context:
                 cfg.carn_star = int(bool(v))
             elif k == "math":
                 cfg.math = {"reference": 0, "fast": 1}[v] if isinstance(v, str) else int(v)
+            elif k == "rounding":
+                cfg.rounding = {"nearest": 0, "stochastic": 1}[v] if isinstance(v, str) else int(v)
             else:
                 if not hasattr(cfg, k):
                     raise TypeError(f"unknown parameter {k}")

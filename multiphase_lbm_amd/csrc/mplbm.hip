@@ -118,6 +118,8 @@ struct Phys {
   int face[6];      // face types
   C face_rho[6];
   int write_macro;
+  int sr;            // stochastic rounding of the stored deviations
+  uint32_t sr_seed;  // per step
 };
 
 // ----------------------------------------------------------------------------
@@ -145,6 +147,36 @@ __device__ __forceinline__ typename Mo::S encode(typename Mo::C v, int i, const 
     const float g = __builtin_fmaf(v, ph.enc_mul[wclass(i)], ph.enc_add);
     if constexpr (Mo::kStorage == MPLBM_FP16S) return __float2half_rn(g);
     else return __float2bfloat16(g);
+  }
+}
+
+// 32-bit integer mix (a bijection with good avalanche), for dither streams
+__host__ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+// Stochastic rounding of the deviation: add u - 1/2 units in the last place
+// of the 16-bit format (u uniform in [0, 1) from the top 24 bits of rnd),
+// then round to nearest, so the stored value is v's deviation rounded up
+// with probability equal to its distance from the value below.
+template <class Mo>
+__device__ __forceinline__ typename Mo::S encode_sr(typename Mo::C v, int i, const Phys<typename Mo::C>& ph,
+                                                   uint32_t rnd) {
+  const float g = __builtin_fmaf(v, ph.enc_mul[wclass(i)], ph.enc_add);
+  // 2^floor(log2|g|) from the exponent bits; the format's unit in the last
+  // place is 2^-10 (FP16, at least 2^-24) or 2^-7 (BF16) of it
+  const float p2 = __uint_as_float(__float_as_uint(g) & 0x7F800000u);
+  const float u = (float)(rnd >> 8) * 0x1p-24f - 0.5f;
+  if constexpr (Mo::kStorage == MPLBM_FP16S) {
+    const float ulp = fmaxf(p2 * 0x1p-10f, 0x1p-24f);
+    return __float2half_rn(__builtin_fmaf(u, ulp, g));
+  } else {
+    return __float2bfloat16(__builtin_fmaf(u, p2 * 0x1p-7f, g));
   }
 }
 
@@ -1054,18 +1086,32 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
     const Rsrc O = make_rsrc(L.fout + (uint64_t)(xw + 1) * kQ * L.ps);
     uint32_t psb_o = psb;
     asm volatile("" : "+s"(psb_o));
+    S e[kQ];
+    if (Mo::kDev && ph.sr) {
+      // dither stream of this node and step: a mixed seed, then a linear
+      // congruential sequence over the populations
+      uint32_t rs = mix32(own ^ ((uint32_t)xw * 0x9E3779B1u) ^ ph.sr_seed);
 #pragma unroll
-    for (int i = 0; i < kQ; ++i) bstore<S>(encode<Mo>(f[i], i, ph), O, own, (uint32_t)i * psb_o);
+      for (int i = 0; i < kQ; ++i) {
+        rs = rs * 1664525u + 1013904223u;
+        if constexpr (Mo::kDev) e[i] = encode_sr<Mo>(f[i], i, ph, rs);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < kQ; ++i) e[i] = encode<Mo>(f[i], i, ph);
+    }
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) bstore<S>(e[i], O, own, (uint32_t)i * psb_o);
     if (xw == 0 || xw == L.lx - 1) {  // ghost copies of the first and last planes
       if (xw == 0) {
         const Rsrc G = make_rsrc(L.fout + (uint64_t)(L.lx + 1) * kQ * L.ps);
 #pragma unroll
-        for (int i = 0; i < kQ; ++i) bstore<S>(encode<Mo>(f[i], i, ph), G, own, (uint32_t)i * psb);
+        for (int i = 0; i < kQ; ++i) bstore<S>(e[i], G, own, (uint32_t)i * psb);
       }
       if (xw == L.lx - 1) {
         const Rsrc G = make_rsrc(L.fout);
 #pragma unroll
-        for (int i = 0; i < kQ; ++i) bstore<S>(encode<Mo>(f[i], i, ph), G, own, (uint32_t)i * psb);
+        for (int i = 0; i < kQ; ++i) bstore<S>(e[i], G, own, (uint32_t)i * psb);
       }
     }
     if (ph.write_macro) {
@@ -1445,6 +1491,8 @@ static Phys<C> make_phys(const mplbm_ctx* ctx, int64_t step, int32_t reverse, ui
   }
   ph.enc_add = -ph.rho_ref * ph.scale;
   ph.write_macro = write_macro ? 1 : 0;
+  ph.sr = c.rounding == MPLBM_ROUND_STOCHASTIC ? 1 : 0;
+  ph.sr_seed = mix32((uint32_t)step * 0x9E3779B9u + (uint32_t)((uint64_t)step >> 32) + 0x632BE5ABu);
   return ph;
 }
 
@@ -1827,6 +1875,13 @@ mplbm_status mplbm_create(const mplbm_config* cfg, mplbm_ctx** out) {
     if (cfg->face_type[f] < 0 || cfg->face_type[f] > 1) return fail(nullptr, MPLBM_ERR_ARG, "bad face type");
   if (cfg->kernel < MPLBM_KERNEL_AUTO || cfg->kernel > MPLBM_KERNEL_COMPACT)
     return fail(nullptr, MPLBM_ERR_ARG, "bad kernel");
+  if (cfg->rounding != MPLBM_ROUND_NEAREST && cfg->rounding != MPLBM_ROUND_STOCHASTIC)
+    return fail(nullptr, MPLBM_ERR_ARG, "bad rounding mode");
+  if (cfg->rounding == MPLBM_ROUND_STOCHASTIC &&
+      ((cfg->storage != MPLBM_FP16S && cfg->storage != MPLBM_BF16S) ||
+       (cfg->kernel != MPLBM_KERNEL_AUTO && cfg->kernel != MPLBM_KERNEL_COMPACT)))
+    return fail(nullptr, MPLBM_ERR_UNSUPPORTED,
+                "stochastic rounding applies to FP16S/BF16S storage with the compact kernel");
 
   mplbm_ctx* ctx = new mplbm_ctx();
   ctx->cfg = *cfg;
@@ -2739,6 +2794,9 @@ mplbm_status mplbm_ensemble_create(const mplbm_config* cfgs, int32_t n, int32_t 
     return ens_fail(nullptr, MPLBM_ERR_ARG, "block_threads must be 256, 512 or 1024");
   if (cfgs[0].kernel == MPLBM_KERNEL_TWOPASS)
     return ens_fail(nullptr, MPLBM_ERR_UNSUPPORTED, "the ensemble runs the compact or the fused kernel");
+  for (int32_t s = 0; s < n; ++s)
+    if (cfgs[s].rounding != MPLBM_ROUND_NEAREST)  // a replayed graph would repeat the dither of one step
+      return ens_fail(nullptr, MPLBM_ERR_UNSUPPORTED, "the ensemble stores with round-to-nearest only");
   const bool compact = cfgs[0].kernel != MPLBM_KERNEL_FUSED;
   if (compact && cfgs[0].storage == MPLBM_FP64 && block_threads > 512)
     return ens_fail(nullptr, MPLBM_ERR_ARG, "FP64 compact ensemble needs block_threads <= 512");

@@ -49,7 +49,7 @@ def test_config_layout_matches_c():
     """The ctypes mirror of mplbm_config has the C struct's size (checked
     against the library's default-filled struct)."""
     from multiphase_lbm_amd import _abi
-    assert ctypes.sizeof(_abi.Config) == 248
+    assert ctypes.sizeof(_abi.Config) == 256 and _abi.Config.rounding.offset == 248
     cfg = _abi.Config()
     _abi.load().mplbm_config_default(ctypes.byref(cfg))
     assert (cfg.lx, cfg.ly, cfg.lz) == (400, 400, 400)
