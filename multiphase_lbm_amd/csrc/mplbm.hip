@@ -886,7 +886,7 @@ struct CLists {
 // least one barrier away from the next write of its slot: during the pulls
 // of plane x+1 the rank regions of planes x-2 .. x+3 and the psi regions of
 // planes x-2 .. x+3 can be in use.
-template <class Mo, bool kFaces, int kThreads>
+template <class Mo, bool kFaces, int kThreads, bool kSR = false>
 __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<typename Mo::C>& ph, const CTiling& T,
                                               const CLists& W, int sg, int tile, unsigned char* smem_raw) {
   using C = typename Mo::C;
@@ -1087,14 +1087,14 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
     uint32_t psb_o = psb;
     asm volatile("" : "+s"(psb_o));
     S e[kQ];
-    if (Mo::kDev && ph.sr) {
+    if constexpr (kSR) {
       // dither stream of this node and step: a mixed seed, then a linear
       // congruential sequence over the populations
       uint32_t rs = mix32(own ^ ((uint32_t)xw * 0x9E3779B1u) ^ ph.sr_seed);
 #pragma unroll
       for (int i = 0; i < kQ; ++i) {
         rs = rs * 1664525u + 1013904223u;
-        if constexpr (Mo::kDev) e[i] = encode_sr<Mo>(f[i], i, ph, rs);
+        e[i] = encode_sr<Mo>(f[i], i, ph, rs);
       }
     } else {
 #pragma unroll
@@ -1155,7 +1155,7 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
   }
 }
 
-template <class Mo, bool kFaces, int kThreads>
+template <class Mo, bool kFaces, int kThreads, bool kSR = false>
 __global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typename Mo::C> ph, CTiling T,
                                                       CLists W) {
   extern __shared__ unsigned char smem_raw[];
@@ -1167,7 +1167,7 @@ __global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typena
   if (b >= T.nblocks) return;  // whole padding block leaves before any barrier
   const int ntile = T.tiles_y * T.tiles_z;
   const int k = b / ntile;
-  compact_march<Mo, kFaces, kThreads>(L, ph, T, W, T.seg0 + k * T.seg_stride, b - k * ntile, smem_raw);
+  compact_march<Mo, kFaces, kThreads, kSR>(L, ph, T, W, T.seg0 + k * T.seg_stride, b - k * ntile, smem_raw);
 }
 
 // Store population i of the fluid node of rank r in plane x of a compact
@@ -1748,24 +1748,38 @@ static mplbm_status launch_step(mplbm_ctx* ctx, int64_t step, int32_t reverse, u
     const size_t shm = compact_lds_bytes<C>(T);
     // the workgroup size is at least the tile's interior count (one kept
     // interior node per thread); 1024 only for non-FP64 storage
-    auto launch = [&](const CTiling& R, bool faces) {
+    // stochastic rounding is a separate instantiation so the round-to-nearest
+    // kernels keep the fused multiply-add-and-convert of the encode
+    auto launch_as = [&](auto sr, const CTiling& R, bool faces) {
+      constexpr bool kSR = decltype(sr)::value;
       const int grid = R.xcd_remap ? ((R.nblocks + 7) / 8) * 8 : R.nblocks;
       if (R.threads == 256) {
         if (faces)
-          hipLaunchKernelGGL((k_compact<Mo, true, 256>), dim3(grid), dim3(256), shm, ctx->stream, L, ph, R, WL);
+          hipLaunchKernelGGL((k_compact<Mo, true, 256, kSR>), dim3(grid), dim3(256), shm, ctx->stream, L, ph, R, WL);
         else
-          hipLaunchKernelGGL((k_compact<Mo, false, 256>), dim3(grid), dim3(256), shm, ctx->stream, L, ph, R, WL);
+          hipLaunchKernelGGL((k_compact<Mo, false, 256, kSR>), dim3(grid), dim3(256), shm, ctx->stream, L, ph, R, WL);
       } else if (R.threads == 512) {
         if (faces)
-          hipLaunchKernelGGL((k_compact<Mo, true, 512>), dim3(grid), dim3(512), shm, ctx->stream, L, ph, R, WL);
+          hipLaunchKernelGGL((k_compact<Mo, true, 512, kSR>), dim3(grid), dim3(512), shm, ctx->stream, L, ph, R, WL);
         else
-          hipLaunchKernelGGL((k_compact<Mo, false, 512>), dim3(grid), dim3(512), shm, ctx->stream, L, ph, R, WL);
+          hipLaunchKernelGGL((k_compact<Mo, false, 512, kSR>), dim3(grid), dim3(512), shm, ctx->stream, L, ph, R, WL);
       } else if constexpr (Mo::kStorage != MPLBM_FP64) {
         if (faces)
-          hipLaunchKernelGGL((k_compact<Mo, true, 1024>), dim3(grid), dim3(1024), shm, ctx->stream, L, ph, R, WL);
+          hipLaunchKernelGGL((k_compact<Mo, true, 1024, kSR>), dim3(grid), dim3(1024), shm, ctx->stream, L, ph, R,
+                             WL);
         else
-          hipLaunchKernelGGL((k_compact<Mo, false, 1024>), dim3(grid), dim3(1024), shm, ctx->stream, L, ph, R, WL);
+          hipLaunchKernelGGL((k_compact<Mo, false, 1024, kSR>), dim3(grid), dim3(1024), shm, ctx->stream, L, ph, R,
+                             WL);
       }
+    };
+    auto launch = [&](const CTiling& R, bool faces) {
+      if constexpr (Mo::kDev) {
+        if (ph.sr) {
+          launch_as(std::true_type{}, R, faces);
+          return;
+        }
+      }
+      launch_as(std::false_type{}, R, faces);
     };
     const bool x_faces_only = ph.any_face && !(ph.face[2] | ph.face[3] | ph.face[4] | ph.face[5]);
     if (x_faces_only && T.nseg >= 3 && T.seg >= 2) {
