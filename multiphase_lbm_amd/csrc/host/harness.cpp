@@ -27,6 +27,7 @@
 #include <fstream>
 #include <iostream>
 #include <map>
+#include <mutex>
 #include <set>
 #include <sstream>
 #include <string>
@@ -37,6 +38,9 @@
 
 namespace {
 
+// last error text; the field-file writer thread may set it while the
+// driver runs, so it is guarded
+std::mutex g_err_mu;
 std::string g_err;
 
 int set_err(const char* fmt, ...) {
@@ -45,6 +49,7 @@ int set_err(const char* fmt, ...) {
   va_start(ap, fmt);
   vsnprintf(buf, sizeof buf, fmt, ap);
   va_end(ap);
+  std::lock_guard<std::mutex> lk(g_err_mu);
   g_err = buf;
   return -1;
 }
@@ -879,7 +884,13 @@ bool MultiphaseLBM::mplbm_parse_scalar_token(std::istream& file, const std::stri
 
 extern "C" {
 
-const char* mplbm_host_last_error(void) { return g_err.c_str(); }
+const char* mplbm_host_last_error(void) {
+  // a copy per calling thread, stable until that thread's next call
+  thread_local std::string copy;
+  std::lock_guard<std::mutex> lk(g_err_mu);
+  copy = g_err;
+  return copy.c_str();
+}
 
 void mplbm_host_input_default(mplbm_input* in) {
   std::memset(in, 0, sizeof *in);
