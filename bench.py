@@ -354,7 +354,8 @@ def drainage_bench(a):
     x = lx-1, y/z periodic.  (1) FP32 and FP16 storage throughput on the full
     domain; (2) FP64, FP32, FP16 from the same start on a 1/8 sub-volume
     (384x256x256) for `--drainage-steps` steps: saturation and density error
-    against FP64.  One JSON line."""
+    against FP64 (FP16 with round-to-nearest and with stochastic rounding).
+    One JSON line."""
     import multiphase_lbm_amd as mp
     thr = 0.5 * (1.932442 + 0.156413)
     out = {"workload": "768x512x512 packing, porosity 0.4, r~U[8,20], inlet rho 2.0 / outlet rho 0.15 on x, "
@@ -372,7 +373,7 @@ def drainage_bench(a):
             ctx.step(0, a.warmup)
             ms = ctx.step_timed(a.warmup, a.steps) / a.steps
             st = ctx.reduce()
-        bpu = 2 * Q * mp.STORAGE_BYTES[mp.STORAGE_NAMES[storage.replace("fast", "")]]
+        bpu = 2 * Q * mp.STORAGE_BYTES[mp.STORAGE_NAMES[BASE_STORAGE.get(storage, storage)]]
         out[storage] = {"ms_per_step": round(ms, 4), "mlups": round(nf / ms / 1e3, 1),
                         "roofline_frac": round(nf * bpu / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
                         "fluid_nodes": nf, "nonfinite": st["nonfinite"]}
@@ -382,7 +383,7 @@ def drainage_bench(a):
     sfluid, srho = drainage_fields(sub, 6, 0.4)
     sfl = sfluid.astype(bool)
     res = {}
-    for storage in ("fp64", "fp32", "fp16s"):
+    for storage in ("fp64", "fp32", "fp16s", "fp16sr"):
         t = time.time()
         with mp.Context(*sub, storage=storage, kernel=a.kernel, **DRAIN) as ctx:
             ctx.set_geometry(sfluid)
@@ -399,7 +400,7 @@ def drainage_bench(a):
         res[storage] = (r, st, time.time() - t)
     ref = res["fp64"][0]
     sv = {"shape": list(sub), "steps": a.drainage_steps, "fp64": {"saturation": float((ref[sfl] > thr).mean())}}
-    for storage in ("fp32", "fp16s"):
+    for storage in ("fp32", "fp16s", "fp16sr"):
         r, st, dt = res[storage]
         sv[storage] = {"saturation": float((r[sfl] > thr).mean()),
                        "saturation_abs_err": abs(float((r[sfl] > thr).mean()) - sv["fp64"]["saturation"]),
