@@ -55,6 +55,7 @@ typedef struct mplbm_input {
   int32_t read_checkpoint;  /* 1: resume from checkpoint_filename (token read_checkpoint) */
   int64_t checkpointPeriod; /* exact checkpoint every N steps to <result_folder>/checkpoint.mplbm */
   char checkpoint_filename[256];
+  int32_t rounding;         /* mplbm_rounding (token rounding nearest|stochastic) */
 } mplbm_input;
 
 /* Statistics of MultiphaseLBM::calc_cluster (src/multiphase_lbm.cu:225-562). */

@@ -623,6 +623,7 @@ struct MultiphaseLBM {
     cfg.storage = in.storage;
     cfg.kernel = in.kernel;
     cfg.math = in.math;
+    cfg.rounding = in.rounding;
     cfg.tau = (float)in.tau;
     cfg.G = (float)in.G;
     cfg.carn_star = in.carn_star;
@@ -871,6 +872,10 @@ bool MultiphaseLBM::mplbm_parse_scalar_token(std::istream& file, const std::stri
   } else if (token == "body_force") file >> in.body_force[0] >> in.body_force[1] >> in.body_force[2];
   else if (token == "host_statistics") file >> in.host_statistics;
   else if (token == "checkpointPeriod") file >> in.checkpointPeriod;
+  else if (token == "rounding") {
+    file >> s;
+    in.rounding = s == "stochastic" ? MPLBM_ROUND_STOCHASTIC : MPLBM_ROUND_NEAREST;
+  }
   else if (token == "read_checkpoint") {
     file >> s;
     std::snprintf(in.checkpoint_filename, sizeof in.checkpoint_filename, "%s", s.c_str());
