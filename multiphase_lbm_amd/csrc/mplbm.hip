@@ -975,7 +975,32 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
   // planes xw, xw+1, xw+2 (xw = xp wrapped), one buffer resource.  A solid
   // source gives the halfway bounce-back B_i = A_opp(i) of the node itself
   // (iteration_kernel.cuh:112-115).
-  auto pull = [&](int xp, int p, C* f, int& gy, int& gz, uint32_t& self) -> C {
+  // face rules (anti-bounce-back of the directions entering through a
+  // pressure face) from the node's own populations (rank offset self)
+  auto apply_faces = [&](int xp, C* f, int gy, int gz, uint32_t self) {
+    if constexpr (kFaces) {
+      const int xw = wrapx(xp);
+      if (on_face(ph, xw, gy, gz, L.lx, L.ly, L.lz)) {
+        const Rsrc R = make_rsrc(L.fin + (uint64_t)xw * kQ * L.ps);
+        const uint32_t own = self + kQ * psb;  // own node, relative to memory plane xw
+        C a[kQ];
+#pragma unroll
+        for (int k = 0; k < kQ; ++k) a[k] = decode<Mo>(bload<S>(R, own + (uint32_t)k * psb, 0), k, ph);
+#pragma unroll
+        for (int i = 1; i < kQ; ++i) {
+          const int fc = entering_face(ph, i, xw, gy, gz, L.lx, L.ly, L.lz);
+          if (fc >= 0) f[i] = abb_value<Mo>(a, i, ph.face_rho[fc]);
+        }
+      }
+    }
+  };
+
+  // Pull of the fluid node at psi position p of plane xp: stored values raw,
+  // populations f (face rules applied), density.  Sources in planes xp-1,
+  // xp, xp+1 lie in memory planes xw, xw+1, xw+2 (xw = xp wrapped), one
+  // buffer resource.  A solid source gives the halfway bounce-back B_i =
+  // A_opp(i) of the node itself (iteration_kernel.cuh:112-115).
+  auto pull = [&](int xp, int p, S* raw, C* f, int& gy, int& gz, uint32_t& self) -> C {
     const int xw = wrapx(xp);
     const int r = (int)__umulhi((uint32_t)p, T.magic_pz);
     const int c = p - r * T.PZ;
@@ -990,9 +1015,7 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
     // use instead of being kept live in scalar registers across the loop
     uint32_t psb_o = psb;
     asm volatile("" : "+s"(psb_o));
-    const uint32_t own = self + kQ * psb_o;  // own node, relative to memory plane xw
     const Rsrc R = make_rsrc(L.fin + (uint64_t)xw * kQ * L.ps);
-    S raw[kQ];
 #pragma unroll
     for (int i = 0; i < kQ; ++i) {
       const uint32_t* mi = ex(i) == 1 ? mm : (ex(i) == -1 ? mq : m0);
@@ -1006,18 +1029,7 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
     }
 #pragma unroll
     for (int i = 0; i < kQ; ++i) f[i] = decode<Mo>(raw[i], i, ph);
-    if constexpr (kFaces) {
-      if (on_face(ph, xw, gy, gz, L.lx, L.ly, L.lz)) {
-        C a[kQ];
-#pragma unroll
-        for (int k = 0; k < kQ; ++k) a[k] = decode<Mo>(bload<S>(R, own + (uint32_t)k * psb, 0), k, ph);
-#pragma unroll
-        for (int i = 1; i < kQ; ++i) {
-          const int fc = entering_face(ph, i, xw, gy, gz, L.lx, L.ly, L.lz);
-          if (fc >= 0) f[i] = abb_value<Mo>(a, i, ph.face_rho[fc]);
-        }
-      }
-    }
+    apply_faces(xp, f, gy, gz, self);
     return density(f, gy, ph);
   };
 
@@ -1027,7 +1039,13 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
   // and position stay in registers (the *N set) until the collision of
   // plane xp one iteration later.  Entries from kThreads on are ring nodes
   // and are pulled first, while only the previous plane's set is live.
-  C fN[kQ];
+  // 16-bit storage keeps the stored values of the kept node, two per
+  // register, and decodes them again (with the face rules) at the collision;
+  // wider storage keeps the populations themselves
+  constexpr bool kPackN = sizeof(S) == 2;
+  constexpr int kNW = kPackN ? (kQ + 1) / 2 : kQ;
+  using NT = std::conditional_t<kPackN, uint32_t, C>;
+  NT fN[kNW];
   C rhoN = (C)0;
   int pN = 0;
   uint32_t ownN = 0;
@@ -1044,19 +1062,33 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
     C* pz = psl + slot(xp) * T.PYZ;
     for (int k = tid + kThreads; k < npull; k += kThreads) {
       const int p = W.pos[first + k];
+      S raw[kQ];
       C f[kQ];
       int gy, gz;
       uint32_t self;
-      const C rho = pull(xp, p, f, gy, gz, self);
+      const C rho = pull(xp, p, raw, f, gy, gz, self);
       pz[p] = psi_of<Mo>(ph, rho);
     }
     hasN = false;
     if (tid < npull) {
       pN = W.pos[first + tid];
       int gy, gz;
-      rhoN = pull(xp, pN, fN, gy, gz, ownN);
+      S raw[kQ];
+      C f[kQ];
+      rhoN = pull(xp, pN, raw, f, gy, gz, ownN);
       pz[pN] = psi_of<Mo>(ph, rhoN);
       hasN = keep && tid < nint;
+      if constexpr (kPackN) {
+#pragma unroll
+        for (int k = 0; k < kNW; ++k) {
+          const uint32_t lo = __builtin_bit_cast(uint16_t, raw[2 * k]);
+          const uint32_t hi = 2 * k + 1 < kQ ? (uint32_t)__builtin_bit_cast(uint16_t, raw[2 * k + 1]) : 0u;
+          fN[k] = lo | (hi << 16);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < kQ; ++i) fN[i] = f[i];
+      }
     }
   };
 
@@ -1136,9 +1168,9 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
     uint2 wpre[kPre];
     if (more) expand_issue(x + 3, wpre);
     // this thread's interior node of plane x, pulled one iteration ago
-    C f[kQ];
+    NT fk[kNW];
 #pragma unroll
-    for (int i = 0; i < kQ; ++i) f[i] = fN[i];
+    for (int k = 0; k < kNW; ++k) fk[k] = fN[k];
     const C rho = rhoN;
     const int p = pN;
     const uint32_t own = ownN;
@@ -1150,7 +1182,18 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
     // (4) collide this thread's fluid interior node of plane x
     if (has) {
       const int r = (int)__umulhi((uint32_t)p, T.magic_pz);
-      collide_node(x, p, f, rho, ycrd[r + 1], zcrd[p - r * T.PZ + 1], own);
+      const int gy = ycrd[r + 1], gz = zcrd[p - r * T.PZ + 1];
+      C f[kQ];
+      if constexpr (kPackN) {
+#pragma unroll
+        for (int i = 0; i < kQ; ++i)
+          f[i] = decode<Mo>(__builtin_bit_cast(S, (uint16_t)(fk[i / 2] >> (16 * (i & 1)))), i, ph);
+        apply_faces(x, f, gy, gz, own);
+      } else {
+#pragma unroll
+        for (int i = 0; i < kQ; ++i) f[i] = fk[i];
+      }
+      collide_node(x, p, f, rho, gy, gz, own);
     }
   }
 }
