@@ -1607,6 +1607,9 @@ static int fused_threads(const Tiling& T) { return ((T.TY * T.TZ + 63) / 64) * 6
 // where the axis allows, so a tile row covers whole rank words), segments of
 // seg planes; about 1024 workgroups or more.
 
+// LDS one workgroup may allocate on gfx950
+static constexpr size_t kMaxGroupLds = 163840;
+
 static mplbm_status choose_ctiling(mplbm_ctx* ctx) {
   const mplbm_config& c = ctx->cfg;
   CTiling T{};
@@ -1634,7 +1637,8 @@ static mplbm_status choose_ctiling(mplbm_ctx* ctx) {
     const size_t csz = c.storage == MPLBM_FP64 ? 8 : 4;
     const size_t lds = kCompactSlots * ((size_t)T.PYZ * csz + (size_t)T.RYZ * 4) + (size_t)(ty + 4 + tz + 4) * 4 +
                        2 * (size_t)T.RYZ * 4;
-    if (lds > 65536) return fail(ctx, MPLBM_ERR_ARG, "tile %dx%d needs %zu bytes of LDS (max 65536)", ty, tz, lds);
+    if (lds > kMaxGroupLds)
+      return fail(ctx, MPLBM_ERR_ARG, "tile %dx%d needs %zu bytes of LDS (max %zu)", ty, tz, lds, kMaxGroupLds);
   }
   const int64_t tiles = (int64_t)T.tiles_y * T.tiles_z;
   if (seg <= 0) {
@@ -1796,24 +1800,19 @@ static mplbm_status launch_step(mplbm_ctx* ctx, int64_t step, int32_t reverse, u
     auto launch_as = [&](auto sr, const CTiling& R, bool faces) {
       constexpr bool kSR = decltype(sr)::value;
       const int grid = R.xcd_remap ? ((R.nblocks + 7) / 8) * 8 : R.nblocks;
+      using KernelPtr = void (*)(Lattice<Mo>, Phys<C>, CTiling, CLists);
+      KernelPtr k = nullptr;
       if (R.threads == 256) {
-        if (faces)
-          hipLaunchKernelGGL((k_compact<Mo, true, 256, kSR>), dim3(grid), dim3(256), shm, ctx->stream, L, ph, R, WL);
-        else
-          hipLaunchKernelGGL((k_compact<Mo, false, 256, kSR>), dim3(grid), dim3(256), shm, ctx->stream, L, ph, R, WL);
+        k = faces ? k_compact<Mo, true, 256, kSR> : k_compact<Mo, false, 256, kSR>;
       } else if (R.threads == 512) {
-        if (faces)
-          hipLaunchKernelGGL((k_compact<Mo, true, 512, kSR>), dim3(grid), dim3(512), shm, ctx->stream, L, ph, R, WL);
-        else
-          hipLaunchKernelGGL((k_compact<Mo, false, 512, kSR>), dim3(grid), dim3(512), shm, ctx->stream, L, ph, R, WL);
+        k = faces ? k_compact<Mo, true, 512, kSR> : k_compact<Mo, false, 512, kSR>;
       } else if constexpr (Mo::kStorage != MPLBM_FP64) {
-        if (faces)
-          hipLaunchKernelGGL((k_compact<Mo, true, 1024, kSR>), dim3(grid), dim3(1024), shm, ctx->stream, L, ph, R,
-                             WL);
-        else
-          hipLaunchKernelGGL((k_compact<Mo, false, 1024, kSR>), dim3(grid), dim3(1024), shm, ctx->stream, L, ph, R,
-                             WL);
+        k = faces ? k_compact<Mo, true, 1024, kSR> : k_compact<Mo, false, 1024, kSR>;
       }
+      if (!k) return;
+      // dynamic LDS above 64 KiB (gfx950 gives a workgroup up to 160 KiB)
+      if (shm > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+      hipLaunchKernelGGL(k, dim3(grid), dim3(R.threads), shm, ctx->stream, L, ph, R, WL);
     };
     auto launch = [&](const CTiling& R, bool faces) {
       if constexpr (Mo::kDev) {
