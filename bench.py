@@ -70,6 +70,9 @@ def parse():
                     help="calc_cluster statistics of the 512^3 two-phase run: device (mplbm_calc_cluster) against "
                          "the host path (field read + host cluster_stats), same results")
     ap.add_argument("--cluster-steps", type=int, default=2000)
+    ap.add_argument("--voxeliser", action="store_true",
+                    help="voxelise the spheres of the 512^3 packing on the device (mplbm_place_spheres) and on the "
+                         "host (place_sphere per sphere), same mask and solid numbers")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain device-to-device copy kernel of 2 GiB (rocprofv3 --pmc calibration)")
     return ap.parse_args()
@@ -459,6 +462,33 @@ def cluster_bench(a, shape, fluid, rho, sph):
                       "cluster_order": dev["cluster_order"], "wet_coord_num": dev["wet_coord_num"]}), flush=True)
 
 
+def voxeliser_bench(shape, fluid, sph):
+    """place_sphere (src/multiphase_lbm.cu:893-927) over the packing's sphere list, as read_positions calls it:
+    the host restatement sphere by sphere against one device call (host arrays in and out, transfers
+    included)."""
+    import multiphase_lbm_amd as mp
+    from multiphase_lbm_amd import harness as H
+    spheres = np.ascontiguousarray(np.asarray(sph, np.float32))
+    m0, s0 = np.ones(shape, np.uint8), np.zeros(shape, np.int64)
+    mp.place_spheres(m0, spheres[:1], 1, s0)  # first device call initialises the runtime
+    times = []
+    for _ in range(3):
+        md, sd = np.ones(shape, np.uint8), np.zeros(shape, np.int64)
+        t = time.perf_counter()
+        mp.place_spheres(md, spheres, 1, sd)
+        times.append(time.perf_counter() - t)
+    mh, sh = np.ones(shape, np.uint8), np.zeros(shape, np.int64)
+    t = time.perf_counter()
+    for k, (x, y, z, r) in enumerate(spheres):
+        H.place_sphere(mh, float(x), float(y), float(z), float(r), sh, k + 1)
+    t_host = time.perf_counter() - t
+    same = bool(np.array_equal(md, mh) and np.array_equal(sd, sh) and np.array_equal(md, fluid.astype(np.uint8)))
+    dev_s = sorted(times)[1]
+    print(json.dumps({"mode": "voxeliser", "workload": f"{shape[0]}^3 packing, {len(spheres)} spheres",
+                      "device_s": round(dev_s, 4), "host_s": round(t_host, 3),
+                      "speedup": round(t_host / dev_s, 1), "results_equal": same}), flush=True)
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -478,6 +508,9 @@ def main():
     nsph = len(sph)
     if a.clusters:
         cluster_bench(a, shape, fluid, rho, sph)
+        return
+    if a.voxeliser:
+        voxeliser_bench(shape, fluid, sph)
         return
     if a.precision:
         precision_study(a, shape, fluid, rho)

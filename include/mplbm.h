@@ -269,6 +269,20 @@ mplbm_status mplbm_set_solid_numbers(mplbm_ctx* ctx, const int64_t* solid_number
 mplbm_status mplbm_calc_cluster(mplbm_ctx* ctx, double rhol_spinodal, double rhog_spinodal, mplbm_clusters* out,
                                  int32_t* liq_label, int32_t* gas_label);
 
+/* Sphere voxeliser on the device: place_sphere (src/multiphase_lbm.cu:
+ * 893-927) for count spheres x, y, z, R (host float[count][4]) in list
+ * order, as read_positions (:1106-1126) places the spheres of a positions
+ * file.  The same float-stepped sample points, dist2 < R^2 test, nearest()
+ * rounding and single periodic wrap as the reference's loop (cells beyond
+ * one wrap are skipped).  is_fluid (host uint8 [x][y][z], lx*ly*lz) is
+ * updated in place: covered cells become 0.  solid_number (host int64
+ * [x][y][z], may be NULL) receives first_id + k for the last sphere k of
+ * the list that covers the cell; other cells keep their value.  Every
+ * coordinate +/- R must lie within +/-2^23 and R <= 510.  Synchronous, on
+ * the calling thread's current device; errors go to mplbm_last_error(NULL). */
+mplbm_status mplbm_place_spheres(int64_t lx, int64_t ly, int64_t lz, const float* spheres, int64_t count,
+                                 int64_t first_id, uint8_t* is_fluid, int64_t* solid_number);
+
 /* Changes the FUSED or COMPACT kernel's tile (see mplbm_config.tile_y) and
  * planes per block; 0 keeps the tuned default for that field.  Used by the
  * autotuner (multiphase_lbm_amd/autotune.py); results do not depend on the

@@ -56,6 +56,8 @@ typedef struct mplbm_input {
   int64_t checkpointPeriod; /* exact checkpoint every N steps to <result_folder>/checkpoint.mplbm */
   char checkpoint_filename[256];
   int32_t rounding;         /* mplbm_rounding (token rounding nearest|stochastic) */
+  int32_t device_voxeliser; /* 1: read_positions and generate_solids voxelise on the
+                               device (mplbm_place_spheres); 0: on the host */
 } mplbm_input;
 
 /* Statistics of MultiphaseLBM::calc_cluster (src/multiphase_lbm.cu:225-562). */
@@ -89,6 +91,11 @@ void mplbm_host_place_sphere(uint8_t* is_fluid, int64_t* solid_number, int64_t l
  * each multiplied by res.  Returns the number of spheres placed or < 0. */
 int64_t mplbm_host_read_positions(const char* path, float res, uint8_t* is_fluid, int64_t* solid_number,
                                   int64_t lx, int64_t ly, int64_t lz, int64_t first_id);
+
+/* mplbm_host_read_positions with the spheres voxelised on the device in
+ * one mplbm_place_spheres call (same mask and solid numbers). */
+int64_t mplbm_host_read_positions_device(const char* path, float res, uint8_t* is_fluid, int64_t* solid_number,
+                                         int64_t lx, int64_t ly, int64_t lz, int64_t first_id);
 
 /* Density initialisation of resize_gpu (src/multiphase_lbm.cu:134-144):
  * srand(1), loops z, y, x, rho0*(1 + IniPerturbRate*(rand()/RAND_MAX - 0.5)). */

@@ -18,7 +18,7 @@ from ._abi import (BF16S, FACE_PERIODIC, FACE_PRESSURE, FP16S, FP32, FP64, KERNE
 
 __all__ = ["Context", "MplbmError", "FP32", "FP64", "FP16S", "BF16S", "KERNEL_AUTO", "KERNEL_FUSED",
            "KERNEL_TWOPASS", "KERNEL_COMPACT", "FACE_PERIODIC", "FACE_PRESSURE", "ROUND_NEAREST",
-           "ROUND_STOCHASTIC", "default_config"]
+           "ROUND_STOCHASTIC", "default_config", "place_spheres"]
 
 KERNEL_NAMES = {"auto": KERNEL_AUTO, "fused": KERNEL_FUSED, "twopass": KERNEL_TWOPASS, "compact": KERNEL_COMPACT}
 
@@ -37,6 +37,24 @@ def default_config() -> _abi.Config:
 
 def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def place_spheres(is_fluid, spheres, first_id=1, solid_number=None):
+    """Voxelise spheres (rows x, y, z, R) on the device into is_fluid
+    (uint8 [x][y][z], updated in place) and solid_number (int64 [x][y][z]
+    or None), as the reference's place_sphere for each sphere in order
+    (mplbm_place_spheres)."""
+    lib = _abi.load()
+    if is_fluid.dtype != np.uint8 or is_fluid.ndim != 3 or not is_fluid.flags.c_contiguous:
+        raise ValueError("is_fluid must be a C-contiguous uint8 [x][y][z] array")
+    if solid_number is not None and (solid_number.dtype != np.int64 or solid_number.shape != is_fluid.shape
+                                     or not solid_number.flags.c_contiguous):
+        raise ValueError("solid_number must be a C-contiguous int64 array of is_fluid's shape")
+    sph = np.ascontiguousarray(np.asarray(spheres, np.float32).reshape(-1, 4))
+    st = lib.mplbm_place_spheres(*[int(v) for v in is_fluid.shape], _ptr(sph), int(sph.shape[0]), int(first_id),
+                                 _ptr(is_fluid), _ptr(solid_number))
+    if st != 0:
+        raise MplbmError(st, lib.mplbm_last_error(None).decode())
 
 
 class Context:

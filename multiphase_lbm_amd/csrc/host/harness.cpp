@@ -98,6 +98,22 @@ void place_sphere(const Dims& d, uint8_t* is_fluid, int64_t* solid_number, float
   }
 }
 
+// values of a positions file (read_positions, :1106-1126): one number per
+// line, x, y, z, r of each sphere in turn, each times res as a double
+// product narrowed to float (:1113-1119); an incomplete last sphere is
+// dropped.  Returns the sphere count or -1.
+int64_t read_positions_values(const char* path, float res, std::vector<float>& v) {
+  std::ifstream f(path);
+  if (!f) return set_err("cannot open positions file %s", path);
+  std::string line;
+  while (std::getline(f, line)) {
+    if (line.empty()) continue;
+    v.push_back((float)(std::stod(line) * (double)res));
+  }
+  v.resize(v.size() / 4 * 4);
+  return (int64_t)(v.size() / 4);
+}
+
 // splitmix64: seeded uniform numbers for the synthetic packings
 struct SplitMix {
   uint64_t s;
@@ -556,22 +572,33 @@ struct MultiphaseLBM {
         float rmin, rmax, xmin, xmax, ymin, ymax, zmin, zmax;
         file >> num >> rmin >> rmax >> xmin >> xmax >> ymin >> ymax >> zmin >> zmax;
         if (!allocated) return set_err("generate_solids before resize");
+        std::vector<float> v;
         for (int i = 0; i < num; ++i) {
           const float r = (rmax - rmin) * ((float)rand() / (float)RAND_MAX) + rmin;
           const float x = (xmax - xmin) * ((float)rand() / (float)RAND_MAX) + xmin;
           const float y = (ymax - ymin) * ((float)rand() / (float)RAND_MAX) + ymin;
           const float z = (zmax - zmin) * ((float)rand() / (float)RAND_MAX) + zmin;
-          solid_count += 1;
-          place_sphere(d, is_fluid.data(), solid_number.data(), x, y, z, r, solid_count);
+          v.insert(v.end(), {x, y, z, r});
         }
+        if (in.device_voxeliser && num > 0) {
+          if (mplbm_place_spheres(d.lx, d.ly, d.lz, v.data(), num, solid_count + 1, is_fluid.data(),
+                                  solid_number.data()) != MPLBM_OK)
+            return set_err("%s", mplbm_last_error(nullptr));
+        } else {
+          for (int i = 0; i < num; ++i)
+            place_sphere(d, is_fluid.data(), solid_number.data(), v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3],
+                         solid_count + 1 + i);
+        }
+        solid_count += std::max(num, 0);
       } else if (token == "read_positions") {
         std::string name;
         file >> name;
         if (!allocated) return set_err("read_positions before resize");
         std::cout << "Reading grain positions and radii from: " << name << "\nwith resolution of: " << in.res
                   << std::endl;
-        const int64_t k = mplbm_host_read_positions(name.c_str(), (float)in.res, is_fluid.data(),
-                                                    solid_number.data(), d.lx, d.ly, d.lz, solid_count + 1);
+        const int64_t k =
+            (in.device_voxeliser ? mplbm_host_read_positions_device : mplbm_host_read_positions)(
+                name.c_str(), (float)in.res, is_fluid.data(), solid_number.data(), d.lx, d.ly, d.lz, solid_count + 1);
         if (k < 0) return -1;
         solid_count += k;
         std::cout << "number of grain = " << solid_count << std::endl;
@@ -871,6 +898,7 @@ bool MultiphaseLBM::mplbm_parse_scalar_token(std::istream& file, const std::stri
     in.face_rho[2 * a + 1] = hi;
   } else if (token == "body_force") file >> in.body_force[0] >> in.body_force[1] >> in.body_force[2];
   else if (token == "host_statistics") file >> in.host_statistics;
+  else if (token == "device_voxeliser") file >> in.device_voxeliser;
   else if (token == "checkpointPeriod") file >> in.checkpointPeriod;
   else if (token == "rounding") {
     file >> s;
@@ -958,23 +986,22 @@ void mplbm_host_place_sphere(uint8_t* is_fluid, int64_t* solid_number, int64_t l
 
 int64_t mplbm_host_read_positions(const char* path, float res, uint8_t* is_fluid, int64_t* solid_number, int64_t lx,
                                   int64_t ly, int64_t lz, int64_t first_id) {
-  std::ifstream f(path);
-  if (!f) return set_err("cannot open positions file %s", path);
+  std::vector<float> v;
+  if (read_positions_values(path, res, v) < 0) return -1;
   const Dims d{lx, ly, lz};
-  std::string line;
-  float v[4] = {0, 0, 0, 0};
-  int i = 0;
-  int64_t count = 0;
-  while (std::getline(f, line)) {
-    if (line.empty()) continue;
-    // value * res: double product narrowed to float (:1113-1119)
-    v[i] = (float)(std::stod(line) * (double)res);
-    if (++i == 4) {
-      place_sphere(d, is_fluid, solid_number, v[0], v[1], v[2], v[3], first_id + count);
-      ++count;
-      i = 0;
-    }
-  }
+  const int64_t count = (int64_t)(v.size() / 4);
+  for (int64_t k = 0; k < count; ++k)
+    place_sphere(d, is_fluid, solid_number, v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3], first_id + k);
+  return count;
+}
+
+int64_t mplbm_host_read_positions_device(const char* path, float res, uint8_t* is_fluid, int64_t* solid_number,
+                                         int64_t lx, int64_t ly, int64_t lz, int64_t first_id) {
+  std::vector<float> v;
+  if (read_positions_values(path, res, v) < 0) return -1;
+  const int64_t count = (int64_t)(v.size() / 4);
+  if (mplbm_place_spheres(lx, ly, lz, v.data(), count, first_id, is_fluid, solid_number) != MPLBM_OK)
+    return set_err("%s", mplbm_last_error(nullptr));
   return count;
 }
 

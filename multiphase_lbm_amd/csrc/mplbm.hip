@@ -3651,3 +3651,141 @@ mplbm_status mplbm_calc_cluster(mplbm_ctx* ctx, double rhol_spinodal, double rho
 }
 
 }  // extern "C"
+
+// ----------------------------------------------------------------------------
+// sphere voxeliser: place_sphere (src/multiphase_lbm.cu:893-927) for a list of
+// spheres, as read_positions (:1106-1126) and generate_solids (:1086-1103)
+// call it once per sphere.  One workgroup per sphere: the float-stepped
+// sample coordinates x - R, x - R + 1, ... <= x + R of each axis go to LDS
+// (the same float additions as the reference's loop), then the workgroup's
+// threads test the sample points of the box.  A covered cell is marked solid
+// and its sphere number raised with an atomic maximum, so the last sphere of
+// the list that covers a cell names it, as with sequential placement.
+
+namespace mplbm {
+
+constexpr int kVoxAxis = 1024;  // sample points per axis of one sphere's box
+constexpr int kVoxThreads = 256;
+
+__global__ __launch_bounds__(kVoxThreads) void k_voxelise(const float4* __restrict__ sph, int64_t count, int lx,
+                                                          int ly, int lz, uint8_t* __restrict__ mask,
+                                                          uint32_t* __restrict__ ids) {
+  __shared__ float ax[3][kVoxAxis];
+  __shared__ int an[3];
+  for (int64_t s = blockIdx.x; s < count; s += gridDim.x) {
+    const float4 q = sph[s];
+    if (threadIdx.x < 3) {
+      const float c = threadIdx.x == 0 ? q.x : (threadIdx.x == 1 ? q.y : q.z);
+      const float lo = c - q.w, hi = c + q.w;
+      int n = 0;
+      // the host checked that every axis has fewer than kVoxAxis samples
+      for (float p = lo; p <= hi && n < kVoxAxis; p += 1.f) ax[threadIdx.x][n++] = p;
+      an[threadIdx.x] = n;
+    }
+    __syncthreads();
+    const int nx = an[0], ny = an[1], nz = an[2];
+    const int nyz = ny * nz;
+    const float R2 = q.w * q.w;
+    const int total = nx * nyz;
+    for (int k = (int)threadIdx.x; k < total; k += kVoxThreads) {
+      const int i = k / nyz;
+      const int r = k - i * nyz;
+      const int j = r / nz;
+      const int l = r - j * nz;
+      const float px = ax[0][i], py = ax[1][j], pz = ax[2][l];
+      const float dx = px - q.x, dy = py - q.y, dz = pz - q.z;
+      const float dist2 = dx * dx + dy * dy + dz * dz;
+      if (!(dist2 < R2)) continue;
+      // nearest() of include/lbm_model.cuh:58 and one periodic wrap
+      int cx = (int)floor((double)px + 0.5), cy = (int)floor((double)py + 0.5), cz = (int)floor((double)pz + 0.5);
+      if (cx < 0) cx += lx;
+      if (cy < 0) cy += ly;
+      if (cz < 0) cz += lz;
+      if (cx >= lx) cx -= lx;
+      if (cy >= ly) cy -= ly;
+      if (cz >= lz) cz -= lz;
+      if (cx < 0 || cy < 0 || cz < 0 || cx >= lx || cy >= ly || cz >= lz) continue;  // beyond one wrap
+      const size_t c = ((size_t)cx * (size_t)ly + (size_t)cy) * (size_t)lz + (size_t)cz;
+      mask[c] = 0;
+      if (ids) atomicMax(&ids[c], (uint32_t)(s + 1));
+    }
+    __syncthreads();  // the next sphere rewrites ax / an
+  }
+}
+
+// solid numbers of the covered cells: first_id - 1 + (list position + 1)
+__global__ void k_vox_numbers(const uint32_t* __restrict__ ids, int64_t* __restrict__ sn, int64_t base, uint64_t n) {
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = ids[c];
+    if (v) sn[c] = base + (int64_t)v;
+  }
+}
+
+}  // namespace mplbm
+
+extern "C" {
+
+mplbm_status mplbm_place_spheres(int64_t lx, int64_t ly, int64_t lz, const float* spheres, int64_t count,
+                                 int64_t first_id, uint8_t* is_fluid, int64_t* solid_number) {
+  using namespace mplbm;
+  if (lx < 1 || ly < 1 || lz < 1 || lx > INT32_MAX || ly > INT32_MAX || lz > INT32_MAX)
+    return fail(nullptr, MPLBM_ERR_ARG, "place_spheres: bad domain %ld x %ld x %ld", (long)lx, (long)ly, (long)lz);
+  if (count < 0 || (count > 0 && !spheres) || !is_fluid)
+    return fail(nullptr, MPLBM_ERR_ARG, "place_spheres: null array or negative count");
+  if (count >= (int64_t)UINT32_MAX)
+    return fail(nullptr, MPLBM_ERR_ARG, "place_spheres: %ld spheres in one call (max %u)", (long)count,
+                UINT32_MAX - 1);
+  for (int64_t s = 0; s < count; ++s) {
+    const float* q = spheres + 4 * s;
+    for (int a = 0; a < 3; ++a) {
+      // the float-stepped axis must end: coordinates well inside float's
+      // integer range and fewer than kVoxAxis samples
+      const float lo = q[a] - q[3], hi = q[a] + q[3];
+      if (!std::isfinite(q[a]) || !std::isfinite(q[3]) || std::fabs(lo) > 8388608.f || std::fabs(hi) > 8388608.f ||
+          (double)hi - (double)lo > (double)(kVoxAxis - 3))
+        return fail(nullptr, MPLBM_ERR_ARG, "place_spheres: sphere %ld (%g, %g, %g, R %g) outside the voxeliser's "
+                    "range (|coordinate| + R <= 2^23, R <= %d)", (long)s, q[0], q[1], q[2], q[3], (kVoxAxis - 3) / 2);
+    }
+  }
+  const uint64_t n = (uint64_t)lx * (uint64_t)ly * (uint64_t)lz;
+  uint8_t* dmask = nullptr;
+  uint32_t* dids = nullptr;
+  int64_t* dsn = nullptr;
+  float4* dsph = nullptr;
+  mplbm_status st = MPLBM_OK;
+  auto check = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess && st == MPLBM_OK)
+      st = fail(nullptr, e == hipErrorOutOfMemory ? MPLBM_ERR_NOMEM : MPLBM_ERR_HIP, "place_spheres: %s: %s", what,
+                hipGetErrorString(e));
+    return st == MPLBM_OK;
+  };
+  if (check(hipMalloc(&dmask, n), "hipMalloc mask") &&
+      check(hipMemcpy(dmask, is_fluid, n, hipMemcpyHostToDevice), "copy mask") &&
+      (count == 0 || (check(hipMalloc(&dsph, (size_t)count * sizeof(float4)), "hipMalloc spheres") &&
+                      check(hipMemcpy(dsph, spheres, (size_t)count * sizeof(float4), hipMemcpyHostToDevice),
+                            "copy spheres"))) &&
+      (!solid_number || (check(hipMalloc(&dids, n * sizeof(uint32_t)), "hipMalloc numbers") &&
+                         check(hipMemset(dids, 0, n * sizeof(uint32_t)), "clear numbers")))) {
+    if (count > 0) {
+      const unsigned grid = (unsigned)std::min<int64_t>(count, 1 << 16);
+      hipLaunchKernelGGL(k_voxelise, dim3(grid), dim3(kVoxThreads), 0, 0, dsph, count, (int)lx, (int)ly, (int)lz,
+                         dmask, dids);
+      check(hipGetLastError(), "k_voxelise");
+    }
+    if (st == MPLBM_OK && check(hipMemcpy(is_fluid, dmask, n, hipMemcpyDeviceToHost), "copy mask back") &&
+        solid_number && count > 0 && check(hipMalloc(&dsn, n * sizeof(int64_t)), "hipMalloc solid numbers") &&
+        check(hipMemcpy(dsn, solid_number, n * sizeof(int64_t), hipMemcpyHostToDevice), "copy solid numbers")) {
+      const unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, 1u << 16);
+      hipLaunchKernelGGL(k_vox_numbers, dim3(grid), dim3(256), 0, 0, dids, dsn, first_id - 1, n);
+      if (check(hipGetLastError(), "k_vox_numbers"))
+        check(hipMemcpy(solid_number, dsn, n * sizeof(int64_t), hipMemcpyDeviceToHost), "copy solid numbers back");
+    }
+  }
+  (void)hipFree(dmask);
+  (void)hipFree(dids);
+  (void)hipFree(dsn);
+  (void)hipFree(dsph);
+  return st;
+}
+
+}  // extern "C"

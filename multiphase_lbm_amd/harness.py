@@ -40,7 +40,7 @@ class Input(ctypes.Structure):
         ("body_force", ctypes.c_double * 3), ("unknown_tokens", ctypes.c_int32),
         ("host_statistics", ctypes.c_int32), ("read_checkpoint", ctypes.c_int32),
         ("checkpointPeriod", I64), ("checkpoint_filename", ctypes.c_char * 256),
-        ("rounding", ctypes.c_int32),
+        ("rounding", ctypes.c_int32), ("device_voxeliser", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -74,6 +74,7 @@ SIGNATURES = {
     "mplbm_host_place_sphere": (None, [VP, VP, I64, I64, I64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                        ctypes.c_float, I64]),
     "mplbm_host_read_positions": (I64, [ctypes.c_char_p, ctypes.c_float, VP, VP, I64, I64, I64, I64]),
+    "mplbm_host_read_positions_device": (I64, [ctypes.c_char_p, ctypes.c_float, VP, VP, I64, I64, I64, I64]),
     "mplbm_host_rho_init": (None, [VP, I64, I64, I64, ctypes.c_float, ctypes.c_float]),
     "mplbm_host_read_densities": (I64, [ctypes.c_char_p, VP, I64, I64, I64]),
     "mplbm_host_read_sample": (ctypes.c_int, [ctypes.c_char_p, VP, I64, I64, I64]),
@@ -125,14 +126,15 @@ def place_sphere(is_fluid, x, y, z, R, solid_number=None, solid_id=0):
     load().mplbm_host_place_sphere(_p(is_fluid), _p(solid_number), lx, ly, lz, x, y, z, R, solid_id)
 
 
-def read_positions(path, shape, res=1.0, is_fluid=None, solid_number=None, first_id=1):
+def read_positions(path, shape, res=1.0, is_fluid=None, solid_number=None, first_id=1, device=False):
+    """read_positions; device=True voxelises on the GPU (mplbm_place_spheres)."""
     lx, ly, lz = shape
     if is_fluid is None:
         is_fluid = np.ones(shape, np.uint8)
     if solid_number is None:
         solid_number = np.zeros(shape, np.int64)
-    n = load().mplbm_host_read_positions(str(path).encode(), res, _p(is_fluid), _p(solid_number), lx, ly, lz,
-                                         first_id)
+    fn = load().mplbm_host_read_positions_device if device else load().mplbm_host_read_positions
+    n = fn(str(path).encode(), res, _p(is_fluid), _p(solid_number), lx, ly, lz, first_id)
     if n < 0:
         raise _err("read_positions")
     return is_fluid, solid_number, n

@@ -79,3 +79,23 @@ def test_compact_tile_limits_report_errors():
         mp.Context(64, 64, 64, kernel="compact", block_threads=300)
     with pytest.raises(mp.MplbmError, match="exceeds 512 interior nodes"):
         mp.Context(64, 64, 64, storage="fp64", kernel="compact", tile_y=24, tile_z=24)
+
+
+def test_place_spheres_refuses_bad_input_before_device_work():
+    """The device voxeliser checks its arguments on the host: a sphere whose
+    float-stepped axis would not end or would not fit the per-axis sample
+    buffer, non-finite values, and arrays of the wrong type."""
+    import numpy as np
+    import multiphase_lbm_amd as mp
+    m = np.ones((8, 8, 8), np.uint8)
+    with pytest.raises(mp.MplbmError, match="outside the voxeliser's range"):
+        mp.place_spheres(m, [[4.0, 4.0, 4.0, 600.0]])
+    with pytest.raises(mp.MplbmError, match="outside the voxeliser's range"):
+        mp.place_spheres(m, [[4.0, float("nan"), 4.0, 2.0]])
+    with pytest.raises(mp.MplbmError, match="outside the voxeliser's range"):
+        mp.place_spheres(m, [[2.0 ** 24, 4.0, 4.0, 2.0]])
+    with pytest.raises(ValueError):
+        mp.place_spheres(m.astype(np.int32), [[4.0, 4.0, 4.0, 2.0]])
+    with pytest.raises(ValueError):
+        mp.place_spheres(m, [[4.0, 4.0, 4.0, 2.0]], solid_number=np.zeros((8, 8, 8), np.int32))
+    assert (m == 1).all()

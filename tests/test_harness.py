@@ -247,3 +247,43 @@ def test_positions_written_and_read_back(tmp_path):
     H.write_positions(p, sph)
     m2, _, n = H.read_positions(p, (32, 32, 32))
     assert n == len(sph) and np.array_equal(m, m2)
+
+
+def _write_sample(path, vox):
+    """A read_sample voxel file: header 'lx ly lz', then one id per line in
+    [x][y][z] order with z fastest, 1 = solid."""
+    fx, fy, fz = vox.shape
+    with open(path, "w") as f:
+        f.write(f"{fx} {fy} {fz}\n")
+        f.write("\n".join(str(int(v)) for v in vox.reshape(-1)))
+        f.write("\n")
+
+
+@pytest.mark.parametrize("file_shape,shape", [((6, 5, 7), (6, 5, 7)), ((12, 10, 14), (6, 5, 7)),
+                                              ((9, 9, 12), (3, 3, 4))])
+def test_read_sample_downsampling(tmp_path, file_shape, shape):
+    """read_sample (src/multiphase_lbm.cu:750-806): ids 1 = solid; every
+    voxel whose x, y and z are multiples of lx_file / lx is kept, in file
+    order, then laid out over the domain [x][y][z]."""
+    rng = np.random.default_rng(sum(file_shape))
+    vox = (rng.random(file_shape) < 0.4).astype(np.int64)
+    p = tmp_path / "sample.txt"
+    _write_sample(p, vox)
+    m = H.read_sample(p, shape)
+    ds = file_shape[0] // shape[0]
+    kept = vox[::ds, ::ds, ::ds]
+    assert kept.shape == shape
+    assert np.array_equal(m, (1 - kept).astype(np.uint8))
+
+
+def test_read_sample_refusals(tmp_path):
+    p = tmp_path / "short.txt"
+    _write_sample(p, np.zeros((4, 4, 4), np.int64))
+    with pytest.raises(RuntimeError, match="smaller than the domain"):
+        H.read_sample(p, (8, 8, 8))
+    q = tmp_path / "trunc.txt"
+    q.write_text("4 4 4\n0\n1\n")
+    with pytest.raises(RuntimeError, match="kept voxels"):
+        H.read_sample(q, (4, 4, 4))
+    with pytest.raises(RuntimeError, match="Cannot open"):
+        H.read_sample(tmp_path / "missing.txt", (4, 4, 4))
