@@ -1,0 +1,95 @@
+"""The 256^3 workload at full size: the reference's 767-sphere positions
+file (tests/golden/example_positions_file: radii 2.5-7.5, porosity 0.50 in
+a 200^3 box) read with resolution 1.28 into 256^3, which keeps porosity ~0.5
+(767 spheres of radii 5-15 in 256^3 would leave porosity ~0.76).  Shan-Chen
+G = -5, tau = 1, rho0 = 0.7 with the reference's srand(1) noise of +/-0.01
+(src/multiphase_lbm.cu:134-144), 10,000 steps.
+
+The oracle cannot run 10,000 steps of 8 M fluid nodes in a test, so the
+checks are the properties that hold at any size:
+* the FP32 compact and full-lattice kernels give the same densities bit for
+  bit (each is bit-identical to the oracle on every small case);
+* the FP64 run (bit-identical to the FP64 oracle on every small case)
+  conserves mass to FP64 rounding;
+* FP32 against FP64: total mass drift, the liquid and vapour coexistence
+  densities (medians of the two phases) and the saturation.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from multiphase_lbm_amd import harness as H
+
+pytestmark = pytest.mark.gpu
+
+import multiphase_lbm_amd as mp  # noqa: E402
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+SHAPE = (256, 256, 256)
+STEPS = 10000
+PHYS = dict(tau=1.0, G=-5.0, rhos=0.7, rhol_spinodal=1.285931, rhog_spinodal=0.323507)
+SPLIT = 0.5 * (1.285931 + 0.323507)  # liquid above, vapour below
+
+
+@pytest.fixture(scope="module")
+def packing():
+    fluid, _, n = H.read_positions(os.path.join(GOLDEN, "example_positions_file"), SHAPE, res=1.28)
+    assert n == 767
+    rho = H.rho_init(SHAPE, 0.7, 0.02 / 0.7)
+    return fluid.astype(bool), rho
+
+
+def _run(fluid, rho, storage, kernel):
+    print(f"case 256^3 {storage} {kernel} {STEPS} steps", flush=True)
+    with mp.Context(*SHAPE, storage=storage, kernel=kernel, **PHYS) as ctx:
+        ctx.set_geometry(fluid)
+        ctx.init_fields(rho.astype(np.float64) if storage == "fp64" else rho)
+        m0 = ctx.reduce()["mass"]
+        for s in range(0, STEPS, 2500):
+            ctx.step(s, 2500, write_macro=s + 2500 == STEPS)
+            ctx.sync()
+            print(f"  {storage} {kernel}: {s + 2500} steps", flush=True)
+        st = ctx.reduce()
+        r, _, _ = ctx.read_fields()
+    return r, st, m0
+
+
+@pytest.fixture(scope="module")
+def runs(packing):
+    fluid, rho = packing
+    return {k: _run(fluid, rho, *k) for k in (("fp64", "compact"), ("fp32", "compact"), ("fp32", "fused"))}
+
+
+def test_fp32_kernels_agree_bit_for_bit(packing, runs):
+    fluid, _ = packing
+    rc, sc, _ = runs[("fp32", "compact")]
+    rf, sf, _ = runs[("fp32", "fused")]
+    assert np.array_equal(rc[fluid], rf[fluid])
+    assert sc["mass"] == sf["mass"]
+
+
+def test_mass_and_coexistence_against_fp64(packing, runs):
+    fluid, _ = packing
+    porosity = float(fluid.mean())
+    r64, s64, m64 = runs[("fp64", "compact")]
+    r32, s32, m32 = runs[("fp32", "compact")]
+    assert s64["nonfinite"] == 0 and s32["nonfinite"] == 0
+    drift64 = s64["mass"] / m64 - 1.0
+    drift32 = s32["mass"] / m32 - 1.0
+    liq64, vap64 = np.median(r64[fluid & (r64 > SPLIT)]), np.median(r64[fluid & (r64 < SPLIT)])
+    liq32, vap32 = np.median(r32[fluid & (r32 > SPLIT)]), np.median(r32[fluid & (r32 < SPLIT)])
+    sat64 = float((r64[fluid] > SPLIT).mean())
+    sat32 = float((r32[fluid] > SPLIT).mean())
+    print(json.dumps({"workload": "256^3, 767-sphere file at resolution 1.28", "porosity": round(porosity, 4),
+                      "steps": STEPS, "mass_drift_fp64": drift64, "mass_drift_fp32": drift32,
+                      "liquid_median": [float(liq64), float(liq32)], "vapour_median": [float(vap64), float(vap32)],
+                      "saturation": [sat64, sat32]}), flush=True)
+    assert 0.45 < porosity < 0.55
+    assert abs(drift64) < 1e-11
+    assert abs(drift32) < 1e-3
+    # two phases separated: bulk liquid near 2 and vapour near 0.15 at G = -5
+    assert liq64 > 1.6 and vap64 < 0.3
+    assert abs(liq32 / liq64 - 1) < 1e-3 and abs(vap32 / vap64 - 1) < 1e-2
+    assert abs(sat32 - sat64) < 0.02
