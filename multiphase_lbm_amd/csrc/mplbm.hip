@@ -2595,15 +2595,47 @@ struct EnsSample {
   Tiling T;
 };
 
-// Persistent launch over the flattened work list; every workgroup walks
-// items blockIdx.x, blockIdx.x + gridDim.x, ...  Sample descriptors are
-// read with wave-uniform addresses (scalar loads).
+// Work queue of a persistent ensemble launch: the next item to take and the
+// number of workgroups that found the queue empty.  Items differ in cost
+// (partial tiles, short segments, porosity), so workgroups take them one at
+// a time instead of a fixed stride.  The last workgroup to finish clears
+// both counters for the next launch, which keeps a captured graph free of
+// memset nodes.
+struct EnsQueue {
+  uint32_t next;
+  uint32_t done;
+};
+
+// Index of the workgroup's next item (the same value in every thread).
+__device__ __forceinline__ int ens_take(EnsQueue* q, int* slot) {
+  if (threadIdx.x == 0) *slot = (int)atomicAdd(&q->next, 1u);
+  __syncthreads();
+  return __builtin_amdgcn_readfirstlane(*slot);
+}
+
+// After the workgroup's last take: the last workgroup clears the queue.
+__device__ __forceinline__ void ens_finish(EnsQueue* q) {
+  if (threadIdx.x == 0) {
+    __threadfence();  // this workgroup's last take precedes its count
+    if (atomicAdd(&q->done, 1u) == gridDim.x - 1) {
+      atomicExch(&q->next, 0u);
+      atomicExch(&q->done, 0u);
+    }
+  }
+}
+
+// Persistent launch over the flattened work list; every workgroup takes
+// items from the work queue until it is empty.  Sample descriptors are read
+// with wave-uniform addresses (scalar loads).
 template <class Mo, int kBlock>
 __global__ __launch_bounds__(kBlock) void k_ensemble(const EnsItem* __restrict__ items, int nitems,
                                                      const EnsSample<Mo>* __restrict__ samples, int parity,
-                                                     int macro) {
+                                                     int macro, EnsQueue* queue) {
   extern __shared__ unsigned char smem_raw[];
-  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+  __shared__ int slot;
+  for (;;) {
+    const int w = ens_take(queue, &slot);
+    if (w >= nitems) break;
     const EnsItem it = items[w];
     const int s = __builtin_amdgcn_readfirstlane(it.sample);
     const EnsSample<Mo>& sd = samples[s];
@@ -2614,6 +2646,7 @@ __global__ __launch_bounds__(kBlock) void k_ensemble(const EnsItem* __restrict__
                           reinterpret_cast<typename Mo::C*>(smem_raw));
     __syncthreads();  // the next item reuses the LDS ring
   }
+  ens_finish(queue);
 }
 
 template <class Mo>
@@ -2629,9 +2662,12 @@ struct EnsSampleC {
 template <class Mo, int kBlock>
 __global__ __launch_bounds__(kBlock) void k_ensemble_c(const EnsItem* __restrict__ items, int nitems,
                                                        const EnsSampleC<Mo>* __restrict__ samples, int parity,
-                                                       int macro) {
+                                                       int macro, EnsQueue* queue) {
   extern __shared__ unsigned char smem_raw[];
-  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+  __shared__ int slot;
+  for (;;) {
+    const int w = ens_take(queue, &slot);
+    if (w >= nitems) break;
     const EnsItem it = items[w];
     const int s = __builtin_amdgcn_readfirstlane(it.sample);
     const EnsSampleC<Mo>& sd = samples[s];
@@ -2642,6 +2678,7 @@ __global__ __launch_bounds__(kBlock) void k_ensemble_c(const EnsItem* __restrict
                                      smem_raw);
     __syncthreads();  // the next item reuses the LDS regions
   }
+  ens_finish(queue);
 }
 
 }  // namespace mplbm
@@ -2656,6 +2693,7 @@ struct mplbm_ensemble {
   std::vector<EnsItem> items;
   EnsItem* d_items = nullptr;
   void* d_samples = nullptr;
+  EnsQueue* d_queue = nullptr;  // cleared at creation, then by each launch's last workgroup
   size_t sample_bytes = 0;
   size_t lds_bytes = 0;
   hipGraph_t graph = nullptr;
@@ -2828,26 +2866,26 @@ static mplbm_status ens_launch(mplbm_ensemble* e, int parity, int macro) {
       const auto* sc = static_cast<const EnsSampleC<Mo>*>(e->d_samples);
       if (e->block == 512)
         hipLaunchKernelGGL((k_ensemble_c<Mo, 512>), dim3(grid), dim3(512), e->lds_bytes, e->stream, e->d_items,
-                           nitems, sc, parity, macro);
+                           nitems, sc, parity, macro, e->d_queue);
       else if (e->block == 256)
         hipLaunchKernelGGL((k_ensemble_c<Mo, 256>), dim3(grid), dim3(256), e->lds_bytes, e->stream, e->d_items,
-                           nitems, sc, parity, macro);
+                           nitems, sc, parity, macro, e->d_queue);
       else if constexpr (Mo::kStorage != MPLBM_FP64)
         hipLaunchKernelGGL((k_ensemble_c<Mo, 1024>), dim3(grid), dim3(1024), e->lds_bytes, e->stream, e->d_items,
-                           nitems, sc, parity, macro);
+                           nitems, sc, parity, macro, e->d_queue);
       ENS_HIP(e, hipGetLastError());
       return MPLBM_OK;
     }
     const auto* samp = static_cast<const EnsSample<Mo>*>(e->d_samples);
     if (e->block == 1024)
       hipLaunchKernelGGL((k_ensemble<Mo, 1024>), dim3(grid), dim3(1024), e->lds_bytes, e->stream, e->d_items,
-                         nitems, samp, parity, macro);
+                         nitems, samp, parity, macro, e->d_queue);
     else if (e->block == 512)
       hipLaunchKernelGGL((k_ensemble<Mo, 512>), dim3(grid), dim3(512), e->lds_bytes, e->stream, e->d_items, nitems,
-                         samp, parity, macro);
+                         samp, parity, macro, e->d_queue);
     else
       hipLaunchKernelGGL((k_ensemble<Mo, 256>), dim3(grid), dim3(256), e->lds_bytes, e->stream, e->d_items, nitems,
-                         samp, parity, macro);
+                         samp, parity, macro, e->d_queue);
     ENS_HIP(e, hipGetLastError());
     return MPLBM_OK;
   });
@@ -2897,9 +2935,11 @@ mplbm_status mplbm_ensemble_create(const mplbm_config* cfgs, int32_t n, int32_t 
   hipError_t he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
   if (he == hipSuccess) he = hipEventCreate(&e->ev0);
   if (he == hipSuccess) he = hipEventCreate(&e->ev1);
+  if (he == hipSuccess) he = hipMalloc((void**)&e->d_queue, sizeof(EnsQueue));
+  if (he == hipSuccess) he = hipMemset(e->d_queue, 0, sizeof(EnsQueue));
   if (he != hipSuccess) {
     mplbm_ensemble_destroy(e);
-    return ens_fail(nullptr, MPLBM_ERR_HIP, "stream/event: %s", hipGetErrorString(he));
+    return ens_fail(nullptr, MPLBM_ERR_HIP, "stream/event/queue: %s", hipGetErrorString(he));
   }
   mplbm_status st = ens_plan(e);
   if (st != MPLBM_OK) {
@@ -2918,6 +2958,7 @@ void mplbm_ensemble_destroy(mplbm_ensemble* e) {
   if (e->graph) (void)hipGraphDestroy(e->graph);
   if (e->d_items) (void)hipFree(e->d_items);
   if (e->d_samples) (void)hipFree(e->d_samples);
+  if (e->d_queue) (void)hipFree(e->d_queue);
   for (mplbm_ctx* c : e->ctx) mplbm_destroy(c);
   if (e->ev0) (void)hipEventDestroy(e->ev0);
   if (e->ev1) (void)hipEventDestroy(e->ev1);
