@@ -92,6 +92,15 @@ typedef enum mplbm_face { MPLBM_FACE_PERIODIC = 0, MPLBM_FACE_PRESSURE = 1 } mpl
  * makes the stored value unbiased; runs stay reproducible. */
 typedef enum mplbm_rounding { MPLBM_ROUND_NEAREST = 0, MPLBM_ROUND_STOCHASTIC = 1 } mplbm_rounding;
 
+/* Shift of the 16-bit deviations (FP16S/BF16S with the COMPACT kernel).
+ * GLOBAL: one shift rho_ref for every population, g_i = (f_i/w_i -
+ * rho_ref)*scale.  NODE: each node's populations are stored as deviations
+ * from that node's density (rounded to the 16-bit type and stored beside
+ * them, 2 B per node), g_i = (f_i/w_i - rho_node)*scale, so the stored
+ * values stay small in both phases of a two-phase run; 4 more bytes per
+ * update, round-to-nearest only. */
+typedef enum mplbm_shift { MPLBM_SHIFT_GLOBAL = 0, MPLBM_SHIFT_NODE = 1 } mplbm_shift;
+
 /* All fields of transfer_parameter (src/iteration_host.cu:16-41) plus the
  * domain size, storage, faces, body force and kernel tiling.  Fill with
  * mplbm_config_default() and override. */
@@ -124,6 +133,7 @@ typedef struct mplbm_config {
   int32_t block_threads;       /* COMPACT: workgroup size 256, 512 or 1024 (0 = tuned default), at
                                   least the tile's interior node count */
   int32_t rounding;            /* mplbm_rounding of the 16-bit forms */
+  int32_t shift;               /* mplbm_shift of the 16-bit forms */
 } mplbm_config;
 
 /* FP64 reductions over fluid nodes (mplbm_reduce). */

@@ -13,12 +13,12 @@ import numpy as np
 
 from . import _abi
 from ._abi import (BF16S, FACE_PERIODIC, FACE_PRESSURE, FP16S, FP32, FP64, KERNEL_AUTO, KERNEL_COMPACT,
-                   KERNEL_FUSED, KERNEL_TWOPASS, ROUND_NEAREST, ROUND_STOCHASTIC, STEP_WRITE_MACRO, STORAGE_BYTES,
-                   STORAGE_NAMES)
+                   KERNEL_FUSED, KERNEL_TWOPASS, ROUND_NEAREST, ROUND_STOCHASTIC, SHIFT_GLOBAL, SHIFT_NODE,
+                   STEP_WRITE_MACRO, STORAGE_BYTES, STORAGE_NAMES)
 
 __all__ = ["Context", "MplbmError", "FP32", "FP64", "FP16S", "BF16S", "KERNEL_AUTO", "KERNEL_FUSED",
            "KERNEL_TWOPASS", "KERNEL_COMPACT", "FACE_PERIODIC", "FACE_PRESSURE", "ROUND_NEAREST",
-           "ROUND_STOCHASTIC", "default_config", "place_spheres"]
+           "ROUND_STOCHASTIC", "SHIFT_GLOBAL", "SHIFT_NODE", "default_config", "place_spheres"]
 
 KERNEL_NAMES = {"auto": KERNEL_AUTO, "fused": KERNEL_FUSED, "twopass": KERNEL_TWOPASS, "compact": KERNEL_COMPACT}
 
@@ -63,8 +63,9 @@ class Context:
     Keyword parameters are fields of mplbm_config (tau, G, carn_star, T_Tc,
     rhol_spinodal, rhog_spinodal, rho_inject_period, rho_increment, rhos,
     inject_position, face_type, face_rho, body_force, rho_ref, dev_scale,
-    tile_y, tile_z, seg_x, math).  storage "fp32fast" is FP32 storage with
-    math="fast"."""
+    tile_y, tile_z, seg_x, math, rounding, shift).  storage "fp32fast" is FP32
+    storage with math="fast"; "fp16sr"/"bf16sr" add rounding="stochastic" and
+    "fp16sn"/"bf16sn" shift="node" to the 16-bit forms."""
 
     def __init__(self, lx, ly, lz, storage="fp32", kernel="auto", **params):
         self.lib = _abi.load()
@@ -74,6 +75,8 @@ class Context:
             storage, params = "fp32", dict(params, math="fast")
         elif storage in ("fp16sr", "bf16sr"):
             storage, params = storage[:-1], dict(params, rounding="stochastic")
+        elif storage in ("fp16sn", "bf16sn"):
+            storage, params = storage[:-1], dict(params, shift="node")
         cfg.storage = STORAGE_NAMES[storage] if isinstance(storage, str) else int(storage)
         cfg.kernel = KERNEL_NAMES[kernel] if isinstance(kernel, str) else int(kernel)
         for k, v in params.items():
@@ -87,6 +90,8 @@ class Context:
                 cfg.math = {"reference": 0, "fast": 1}[v] if isinstance(v, str) else int(v)
             elif k == "rounding":
                 cfg.rounding = {"nearest": 0, "stochastic": 1}[v] if isinstance(v, str) else int(v)
+            elif k == "shift":
+                cfg.shift = {"global": 0, "node": 1}[v] if isinstance(v, str) else int(v)
             else:
                 if not hasattr(cfg, k):
                     raise TypeError(f"unknown parameter {k}")

@@ -19,6 +19,7 @@ MATH_REFERENCE, MATH_FAST = 0, 1
 BB_WRITE, BB_PULL = 0, 1
 STEP_WRITE_MACRO = 1
 ROUND_NEAREST, ROUND_STOCHASTIC = 0, 1
+SHIFT_GLOBAL, SHIFT_NODE = 0, 1
 
 STORAGE_NAMES = {"fp32": FP32, "fp64": FP64, "fp16s": FP16S, "bf16s": BF16S}
 STORAGE_BYTES = {FP32: 4, FP64: 8, FP16S: 2, BF16S: 2}
@@ -39,7 +40,7 @@ class Config(ctypes.Structure):
         ("tile_y", ctypes.c_int32), ("tile_z", ctypes.c_int32),
         ("seg_x", ctypes.c_int32), ("plane_pad", ctypes.c_int32),
         ("bounce_back", ctypes.c_int32), ("block_threads", ctypes.c_int32),
-        ("rounding", ctypes.c_int32),
+        ("rounding", ctypes.c_int32), ("shift", ctypes.c_int32),
     ]
 
 

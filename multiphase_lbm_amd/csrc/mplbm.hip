@@ -109,6 +109,7 @@ struct Phys {
   C inject_ylim;    // inject_position * ly
   C g[3];           // body acceleration
   float rho_ref, scale, inv_scale;  // deviation storage
+  int node_shift;                    // MPLBM_SHIFT_NODE: per-node shifts in Lattice::sin/sout
   float dec_mul[3], dec_add[3];      // per weight class: f = g * dec_mul + dec_add
   float enc_mul[3], enc_add;         // g = f * enc_mul + enc_add
   int inject_now;   // step % rho_inject_period == 0
@@ -148,6 +149,34 @@ __device__ __forceinline__ typename Mo::S encode(typename Mo::C v, int i, const 
     if constexpr (Mo::kStorage == MPLBM_FP16S) return __float2half_rn(g);
     else return __float2bfloat16(g);
   }
+}
+
+// Node-shifted form: w_i (g / scale + shift) and (f / w_i - shift) * scale,
+// shift the 16-bit stored density of the node that wrote the population.
+template <class Mo>
+__device__ __forceinline__ float shift_value(typename Mo::S s) {
+  if constexpr (Mo::kStorage == MPLBM_FP16S) return __half2float(s);
+  else if constexpr (Mo::kStorage == MPLBM_BF16S) return __bfloat162float(s);
+  else return (float)s;
+}
+template <class Mo>
+__device__ __forceinline__ typename Mo::S shift_of(float rho) {
+  if constexpr (Mo::kStorage == MPLBM_FP16S) return __float2half_rn(rho);
+  else if constexpr (Mo::kStorage == MPLBM_BF16S) return __float2bfloat16(rho);
+  else return (typename Mo::S)rho;
+}
+template <class Mo>
+__device__ __forceinline__ typename Mo::C decode_at(typename Mo::S s, int i, const Phys<typename Mo::C>& ph,
+                                                    float shift) {
+  const float g = shift_value<Mo>(s);
+  return __builtin_fmaf(g, ph.dec_mul[wclass(i)], weight<float>(i) * shift);
+}
+// enc_add = -shift * scale of the writing node
+template <class Mo>
+__device__ __forceinline__ typename Mo::S encode_at(typename Mo::C v, int i, const Phys<typename Mo::C>& ph,
+                                                    float enc_add) {
+  const float g = __builtin_fmaf((float)v, ph.enc_mul[wclass(i)], enc_add);
+  return shift_of<Mo>(g);
 }
 
 // 32-bit integer mix (a bijection with good avalanche), for dither streams
@@ -386,6 +415,11 @@ struct Lattice {
   typename Mo::M* pres;
   typename Mo::M* u;     // 3 planes of ncell
   typename Mo::C* psi;   // two-pass psi field
+  // node shifts (MPLBM_SHIFT_NODE, compact layout): one per fluid node (see
+  // sidx), read from sin (written with fin), written to sout (with fout);
+  // null in the global-shift form
+  const typename Mo::S* sin;
+  typename Mo::S* sout;
   // element index of population i of cell c (plane-major layout)
   __device__ __forceinline__ uint64_t pix(int i, uint32_t c) const {
     const uint32_t x = c / plane;
@@ -402,7 +436,9 @@ struct Lattice {
 };
 
 // Compact layout.  Only fluid nodes are stored.  The fluid nodes of an
-// x-plane are numbered in (y, z) raster order (their plane rank r); population
+// x-plane are numbered 32-wide z block by block, rows of y within a block, z
+// within a row (their plane rank r; the rank word of (x, y, block) holds the
+// block's fluid bits of that row and the rank of its first node); population
 // i of the node of rank r in plane x lives at element ((x+1)*19 + i)*ps + r,
 // ps >= the largest plane count.  Memory planes 0 and lx+1 are copies of
 // planes lx-1 and 0 (kept current by the step kernel), so the three source
@@ -417,6 +453,12 @@ __device__ __forceinline__ bool rw_fluid(const uint2* rw, uint32_t W, int ly, in
 template <class Mo>
 __device__ __forceinline__ uint64_t cpix(const Lattice<Mo>& L, int i, int mplane, uint32_t rank) {
   return ((uint64_t)mplane * kQ + i) * L.ps + rank;
+}
+// element of the node shift of the fluid node of rank `rank` in plane x
+// (the populations' plane-rank numbering, no ghost planes)
+template <class Mo>
+__device__ __forceinline__ uint64_t sidx(const Lattice<Mo>& L, int x, uint32_t rank) {
+  return (uint64_t)x * L.ps + rank;
 }
 
 struct Tiling {
@@ -900,7 +942,7 @@ struct CLists {
 // least one barrier away from the next write of its slot: during the pulls
 // of plane x+1 the rank regions of planes x-2 .. x+3 and the psi regions of
 // planes x-2 .. x+3 can be in use.
-template <class Mo, bool kFaces, int kThreads, bool kSR = false>
+template <class Mo, bool kFaces, int kThreads, bool kSR = false, bool kNS = false>
 __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<typename Mo::C>& ph, const CTiling& T,
                                               const CLists& W, int sg, int tile, unsigned char* smem_raw) {
   using C = typename Mo::C;
@@ -911,6 +953,7 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
   int* zcrd = ycrd + (T.TYi + 4);                                            // TZi + 4 columns
   uint32_t* qtab = reinterpret_cast<uint32_t*>(zcrd + (T.TZi + 4));          // RYZ: rank word << 5 | bit
   int* qpsi = reinterpret_cast<int*>(qtab + T.RYZ);                          // RYZ: psi index or -1
+  typename Mo::S* shr = reinterpret_cast<typename Mo::S*>(qpsi + T.RYZ);     // kSlots x RYZ node shifts (kNS)
 
   const int tyi = tile / T.tiles_z;
   const int tzi = tile - tyi * T.tiles_z;
@@ -952,10 +995,16 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
   // overlaps the pulls that follow), expand_finish writes the region.
   constexpr int kPre = 2;
   auto rw_word = [&](const uint2* rwp, int q) { return rwp[qtab[q] >> 5]; };
+  // the node shift of a fluid cell of the rank region goes to LDS with its
+  // rank (kNS)
   auto put_rank = [&](int xp, int q, uint2 w) {
     const uint32_t bit = qtab[q] & 31u;
     const bool fl = (w.x >> bit) & 1u;
-    rm[slot(xp) * T.RYZ + q] = fl ? (w.y + __popc(w.x & ((1u << bit) - 1u))) * sz : ~0u;
+    const uint32_t rank = w.y + __popc(w.x & ((1u << bit) - 1u));
+    rm[slot(xp) * T.RYZ + q] = fl ? rank * sz : ~0u;
+    if constexpr (kNS) {
+      if (fl) shr[slot(xp) * T.RYZ + q] = L.sin[sidx(L, wrapx(xp), rank)];
+    }
     if (!fl) {
       const int pi = qpsi[q];
       if (pi >= 0) psl[slot(xp) * T.PYZ + pi] = psi_solid;
@@ -991,7 +1040,7 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
   // (iteration_kernel.cuh:112-115).
   // face rules (anti-bounce-back of the directions entering through a
   // pressure face) from the node's own populations (rank offset self)
-  auto apply_faces = [&](int xp, C* f, int gy, int gz, uint32_t self) {
+  auto apply_faces = [&](int xp, C* f, int gy, int gz, uint32_t self, float osh) {
     if constexpr (kFaces) {
       const int xw = wrapx(xp);
       if (on_face(ph, xw, gy, gz, L.lx, L.ly, L.lz)) {
@@ -999,7 +1048,11 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
         const uint32_t own = self + kQ * psb;  // own node, relative to memory plane xw
         C a[kQ];
 #pragma unroll
-        for (int k = 0; k < kQ; ++k) a[k] = decode<Mo>(bload<S>(R, own + (uint32_t)k * psb, 0), k, ph);
+        for (int k = 0; k < kQ; ++k) {
+          const S v = bload<S>(R, own + (uint32_t)k * psb, 0);
+          if constexpr (kNS) a[k] = decode_at<Mo>(v, k, ph, osh);
+          else a[k] = decode<Mo>(v, k, ph);
+        }
 #pragma unroll
         for (int i = 1; i < kQ; ++i) {
           const int fc = entering_face(ph, i, xw, gy, gz, L.lx, L.ly, L.lz);
@@ -1007,6 +1060,29 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
         }
       }
     }
+  };
+
+  // Node-shifted form: decode the 19 stored values pulled for the node at
+  // rank-region position q0 of plane xp, each with the shift of the node
+  // that wrote it (the source, or the node itself for a bounce-back), from
+  // the shift regions of planes xp-1, xp, xp+1.  Returns the node's own shift.
+  auto decode_shifted = [&](int xp, int q0, const S* raw, C* f) -> float {
+    const uint32_t* m0 = rm + slot(xp) * T.RYZ + q0;
+    const uint32_t* mm = rm + slot(xp - 1) * T.RYZ + q0;
+    const uint32_t* mq = rm + slot(xp + 1) * T.RYZ + q0;
+    const S* s0 = shr + slot(xp) * T.RYZ + q0;
+    const S* sm = shr + slot(xp - 1) * T.RYZ + q0;
+    const S* sq = shr + slot(xp + 1) * T.RYZ + q0;
+    const float own = shift_value<Mo>(s0[0]);
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) {
+      const uint32_t* mi = ex(i) == 1 ? mm : (ex(i) == -1 ? mq : m0);
+      const S* si = ex(i) == 1 ? sm : (ex(i) == -1 ? sq : s0);
+      const int d = -ey(i) * T.RZ - ez(i);
+      const float sh = mi[d] != ~0u ? shift_value<Mo>(si[d]) : own;
+      f[i] = decode_at<Mo>(raw[i], i, ph, sh);
+    }
+    return own;
   };
 
   // Pull of the fluid node at psi position p of plane xp: stored values raw,
@@ -1041,9 +1117,14 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
       const uint32_t k = fl ? (uint32_t)((1 - ex(i)) * kQ + i) : (uint32_t)(kQ + opp(i));
       raw[i] = bload<S>(R, base + __umul24(k, psb_o), 0);
     }
+    float osh = 0.f;
+    if constexpr (kNS) {
+      osh = decode_shifted(xp, q0, raw, f);
+    } else {
 #pragma unroll
-    for (int i = 0; i < kQ; ++i) f[i] = decode<Mo>(raw[i], i, ph);
-    apply_faces(xp, f, gy, gz, self);
+      for (int i = 0; i < kQ; ++i) f[i] = decode<Mo>(raw[i], i, ph);
+    }
+    apply_faces(xp, f, gy, gz, self, osh);
     return density(f, gy, ph);
   };
 
@@ -1142,6 +1223,13 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
         rs = rs * 1664525u + 1013904223u;
         e[i] = encode_sr<Mo>(f[i], i, ph, rs);
       }
+    } else if constexpr (kNS) {
+      // the node's shift: its density in the 16-bit type, stored beside it
+      const S sh = shift_of<Mo>((float)rho);
+      const float enc_add = -shift_value<Mo>(sh) * ph.scale;
+#pragma unroll
+      for (int i = 0; i < kQ; ++i) e[i] = encode_at<Mo>(f[i], i, ph, enc_add);
+      L.sout[sidx(L, xw, own / sz)] = sh;
     } else {
 #pragma unroll
       for (int i = 0; i < kQ; ++i) e[i] = encode<Mo>(f[i], i, ph);
@@ -1199,10 +1287,18 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
       const int gy = ycrd[r + 1], gz = zcrd[p - r * T.PZ + 1];
       C f[kQ];
       if constexpr (kPackN) {
+        float osh = 0.f;
+        if constexpr (kNS) {
+          S raw[kQ];
 #pragma unroll
-        for (int i = 0; i < kQ; ++i)
-          f[i] = decode<Mo>(__builtin_bit_cast(S, (uint16_t)(fk[i / 2] >> (16 * (i & 1)))), i, ph);
-        apply_faces(x, f, gy, gz, own);
+          for (int i = 0; i < kQ; ++i) raw[i] = __builtin_bit_cast(S, (uint16_t)(fk[i / 2] >> (16 * (i & 1))));
+          osh = decode_shifted(x, (r + 1) * T.RZ + (p - r * T.PZ + 1), raw, f);
+        } else {
+#pragma unroll
+          for (int i = 0; i < kQ; ++i)
+            f[i] = decode<Mo>(__builtin_bit_cast(S, (uint16_t)(fk[i / 2] >> (16 * (i & 1)))), i, ph);
+        }
+        apply_faces(x, f, gy, gz, own, osh);
       } else {
 #pragma unroll
         for (int i = 0; i < kQ; ++i) f[i] = fk[i];
@@ -1212,9 +1308,18 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
   }
 }
 
-template <class Mo, bool kFaces, int kThreads, bool kSR = false>
-__global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typename Mo::C> ph, CTiling T,
-                                                      CLists W) {
+// Occupancy the node-shifted 16-bit kernels without faces are held to: 6
+// waves per SIMD (80 VGPRs), three 512-thread workgroups per CU like the
+// global-shift kernel (they would take 81 and drop to two); 1 leaves the
+// register budget to the compiler (the face variants would spill at 80).
+template <bool kNS, bool kFaces>
+constexpr int compact_min_waves() {
+  return kNS && !kFaces ? 6 : 1;
+}
+
+template <class Mo, bool kFaces, int kThreads, bool kSR = false, bool kNS = false>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(compact_min_waves<kNS, kFaces>())))
+void k_compact(Lattice<Mo> L, Phys<typename Mo::C> ph, CTiling T, CLists W) {
   extern __shared__ unsigned char smem_raw[];
   int b = blockIdx.x;
   if (T.xcd_remap) {
@@ -1224,7 +1329,7 @@ __global__ __launch_bounds__(kThreads) void k_compact(Lattice<Mo> L, Phys<typena
   if (b >= T.nblocks) return;  // whole padding block leaves before any barrier
   const int ntile = T.tiles_y * T.tiles_z;
   const int k = b / ntile;
-  compact_march<Mo, kFaces, kThreads, kSR>(L, ph, T, W, T.seg0 + k * T.seg_stride, b - k * ntile, smem_raw);
+  compact_march<Mo, kFaces, kThreads, kSR, kNS>(L, ph, T, W, T.seg0 + k * T.seg_stride, b - k * ntile, smem_raw);
 }
 
 // Store population i of the fluid node of rank r in plane x of a compact
@@ -1262,7 +1367,8 @@ __global__ void k_links(const uint8_t* __restrict__ fluid, uint32_t* __restrict_
 
 // multiphaselbm_init (include/iteration_kernel.cuh:9-29)
 template <class Mo>
-__global__ void k_init(Lattice<Mo> L, typename Mo::S* fother, Phys<typename Mo::C> ph, const double* rho_in) {
+__global__ void k_init(Lattice<Mo> L, typename Mo::S* fother, typename Mo::S* sother, Phys<typename Mo::C> ph,
+                       const double* rho_in) {
   using C = typename Mo::C;
   const uint32_t c = blockIdx.x * 256u + threadIdx.x;
   if (c >= L.ncell) return;
@@ -1279,11 +1385,30 @@ __global__ void k_init(Lattice<Mo> L, typename Mo::S* fother, Phys<typename Mo::
     int x, y, z;
     cell_coords(L, c, x, y, z);
     const uint32_t r = plane_rank(L.rw, L.W, L.ly, x, y, z);
+    bool shifted = false;
+    if constexpr (Mo::kDev) {
+      if (ph.node_shift) {
+        // both lattices and both shift arrays: shift = the node's density
+        const typename Mo::S sh = shift_of<Mo>((float)rho);
+        const float enc_add = -shift_value<Mo>(sh) * ph.scale;
+        L.sout[sidx(L, x, r)] = sh;
+        sother[sidx(L, x, r)] = sh;
 #pragma unroll
-    for (int i = 0; i < kQ; ++i) {
-      const typename Mo::S v = encode<Mo>(weight<C>(i) * rho, i, ph);
-      compact_store_all(L, L.fout, i, x, r, v);
-      compact_store_all(L, fother, i, x, r, v);
+        for (int i = 0; i < kQ; ++i) {
+          const typename Mo::S v = encode_at<Mo>(weight<C>(i) * rho, i, ph, enc_add);
+          compact_store_all(L, L.fout, i, x, r, v);
+          compact_store_all(L, fother, i, x, r, v);
+        }
+        shifted = true;
+      }
+    }
+    if (!shifted) {
+#pragma unroll
+      for (int i = 0; i < kQ; ++i) {
+        const typename Mo::S v = encode<Mo>(weight<C>(i) * rho, i, ph);
+        compact_store_all(L, L.fout, i, x, r, v);
+        compact_store_all(L, fother, i, x, r, v);
+      }
     }
   }
   if (fluid) {
@@ -1303,6 +1428,27 @@ __global__ void k_init(Lattice<Mo> L, typename Mo::S* fother, Phys<typename Mo::
 // decode the current lattice into [cell][19] doubles / encode back
 // element index of population i of cell c in either layout (compact: fluid
 // cells only)
+// shift of fluid cell c's stored populations (node-shifted form; 0 otherwise)
+template <class Mo>
+__device__ __forceinline__ float own_shift(const Lattice<Mo>& L, const Phys<typename Mo::C>& ph, uint32_t c) {
+  if constexpr (Mo::kDev) {
+    if (ph.node_shift) {
+      int x, y, z;
+      cell_coords(L, c, x, y, z);
+      return shift_value<Mo>(L.sin[sidx(L, x, plane_rank(L.rw, L.W, L.ly, x, y, z))]);
+    }
+  }
+  return 0.f;
+}
+template <class Mo>
+__device__ __forceinline__ typename Mo::C decode_own(typename Mo::S s, int i, const Phys<typename Mo::C>& ph,
+                                                     float shift) {
+  if constexpr (Mo::kDev) {
+    if (ph.node_shift) return decode_at<Mo>(s, i, ph, shift);
+  }
+  return decode<Mo>(s, i, ph);
+}
+
 template <class Mo>
 __device__ __forceinline__ uint64_t any_pix(const Lattice<Mo>& L, int i, uint32_t c) {
   if (!L.compact) return L.pix(i, c);
@@ -1320,8 +1466,9 @@ __global__ void k_read_pops(Lattice<Mo> L, Phys<typename Mo::C> ph, double* out)
     for (int i = 0; i < kQ; ++i) out[(uint64_t)c * kQ + i] = 0.0;
     return;
   }
+  const float sh = own_shift(L, ph, c);
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) out[(uint64_t)c * kQ + i] = (double)decode<Mo>(L.fin[any_pix(L, i, c)], i, ph);
+  for (int i = 0; i < kQ; ++i) out[(uint64_t)c * kQ + i] = (double)decode_own<Mo>(L.fin[any_pix(L, i, c)], i, ph, sh);
 }
 
 template <class Mo>
@@ -1338,6 +1485,21 @@ __global__ void k_write_pops(Lattice<Mo> L, Phys<typename Mo::C> ph, const doubl
   int x, y, z;
   cell_coords(L, c, x, y, z);
   const uint32_t r = plane_rank(L.rw, L.W, L.ly, x, y, z);
+  if constexpr (Mo::kDev) {
+    if (ph.node_shift) {
+      // shift: the node's density, summed in float in direction order
+      float rho = 0.f;
+#pragma unroll
+      for (int i = 0; i < kQ; ++i) rho += (float)in[(uint64_t)c * kQ + i];
+      const typename Mo::S sh = shift_of<Mo>(rho);
+      const float enc_add = -shift_value<Mo>(sh) * ph.scale;
+      L.sout[sidx(L, x, r)] = sh;
+#pragma unroll
+      for (int i = 0; i < kQ; ++i)
+        compact_store_all(L, L.fout, i, x, r, encode_at<Mo>((C)in[(uint64_t)c * kQ + i], i, ph, enc_add));
+      return;
+    }
+  }
 #pragma unroll
   for (int i = 0; i < kQ; ++i) compact_store_all(L, L.fout, i, x, r, encode<Mo>((C)in[(uint64_t)c * kQ + i], i, ph));
 }
@@ -1370,6 +1532,23 @@ __global__ void k_plane_raw(Lattice<Mo> L, int x, typename Mo::S* buf, int scatt
   for (int i = 0; i < kQ; ++i) compact_store_all(L, L.fout, i, x, rk, b[i]);
 }
 
+// Node shifts of plane x, [y][z], solid cells 0: gather from sin or scatter
+// to sout (node-shifted compact form).
+template <class Mo>
+__global__ void k_plane_shift(Lattice<Mo> L, int x, typename Mo::S* buf, int scatter) {
+  using S = typename Mo::S;
+  const uint32_t r = blockIdx.x * 256u + threadIdx.x;
+  if (r >= L.plane) return;
+  const uint32_t c = (uint32_t)x * L.plane + r;
+  const bool fluid = !(L.link[c] & 1u);
+  const int y = (int)(r / (uint32_t)L.lz), z = (int)(r - (uint32_t)y * L.lz);
+  if (!scatter) {
+    buf[r] = fluid ? L.sin[sidx(L, x, plane_rank(L.rw, L.W, L.ly, x, y, z))] : S{};
+    return;
+  }
+  if (fluid) L.sout[sidx(L, x, plane_rank(L.rw, L.W, L.ly, x, y, z))] = buf[r];
+}
+
 // ----------------------------------------------------------------------------
 // FP64 reductions: per-block partials, then one block folds them in order
 
@@ -1387,9 +1566,10 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(Lattice<Mo> L, Phys<type
   for (uint32_t c = blockIdx.x * kRedThreads + threadIdx.x; c < L.ncell; c += gridDim.x * kRedThreads) {
     if (L.link[c] & 1u) continue;
     double s = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
+    const float sh = own_shift(L, ph, c);
 #pragma unroll
     for (int i = 0; i < kQ; ++i) {
-      const double f = (double)decode<Mo>(L.fin[any_pix(L, i, c)], i, ph);
+      const double f = (double)decode_own<Mo>(L.fin[any_pix(L, i, c)], i, ph, sh);
       if (!(f - f == 0.0)) acc[15] += 1.0;
       s += f;
       mx += f * ex(i);
@@ -1445,6 +1625,7 @@ struct mplbm_ctx {
   uint64_t ps = 0;  // population sub-plane stride (elements)
   int bytes_pop = 4;
   void* lat[2] = {nullptr, nullptr};  // population lattices
+  void* shift[2] = {nullptr, nullptr};  // node shifts with each lattice (MPLBM_SHIFT_NODE)
   int cur = 0;                        // lattice holding the newest post-collision state
   uint32_t* link = nullptr;
   uint8_t* mask = nullptr;
@@ -1472,6 +1653,7 @@ struct mplbm_ctx {
   CTiling ctiling{};
   std::vector<uint8_t> hmask;   // host copy of the fluid mask (list rebuilds, checkpoints)
   size_t lat_bytes = 0;
+  size_t shift_bytes = 0;  // one node-shift array
   // cluster statistics (allocated at the first mplbm_calc_cluster)
   int32_t* cl_parent = nullptr;  // union-find parents, then cluster labels
   int32_t* cl_size = nullptr;    // component sizes at roots, then cluster numbers
@@ -1538,6 +1720,7 @@ static Phys<C> make_phys(const mplbm_ctx* ctx, int64_t step, int32_t reverse, ui
     ph.any_face |= c.face_type[f] != 0;
   }
   ph.rho_ref = ctx->rho_ref;
+  ph.node_shift = c.shift == MPLBM_SHIFT_NODE ? 1 : 0;
   ph.scale = ctx->scale;
   ph.inv_scale = 1.0f / ctx->scale;
   for (int k = 0; k < 3; ++k) {
@@ -1558,6 +1741,8 @@ static Lattice<Mo> make_lattice(const mplbm_ctx* ctx, int in, int out) {
   Lattice<Mo> L{};
   L.fin = static_cast<const typename Mo::S*>(ctx->lat[in]);
   L.fout = static_cast<typename Mo::S*>(ctx->lat[out]);
+  L.sin = static_cast<const typename Mo::S*>(ctx->shift[in]);
+  L.sout = static_cast<typename Mo::S*>(ctx->shift[out]);
   L.link = ctx->link;
   L.rho = static_cast<typename Mo::M*>(ctx->rho);
   L.pres = static_cast<typename Mo::M*>(ctx->pres);
@@ -1650,7 +1835,7 @@ static mplbm_status choose_ctiling(mplbm_ctx* ctx) {
   {
     const size_t csz = c.storage == MPLBM_FP64 ? 8 : 4;
     const size_t lds = kCompactSlots * ((size_t)T.PYZ * csz + (size_t)T.RYZ * 4) + (size_t)(ty + 4 + tz + 4) * 4 +
-                       2 * (size_t)T.RYZ * 4;
+                       2 * (size_t)T.RYZ * 4 + (c.shift == MPLBM_SHIFT_NODE ? kCompactSlots * (size_t)T.RYZ * 2 : 0);
     if (lds > kMaxGroupLds)
       return fail(ctx, MPLBM_ERR_ARG, "tile %dx%d needs %zu bytes of LDS (max %zu)", ty, tz, lds, kMaxGroupLds);
   }
@@ -1685,10 +1870,11 @@ static mplbm_status choose_ctiling(mplbm_ctx* ctx) {
   return MPLBM_OK;
 }
 
+// shift_bytes: bytes of a node shift in the node-shifted form, else 0
 template <class C>
-static size_t compact_lds_bytes(const CTiling& T) {
+static size_t compact_lds_bytes(const CTiling& T, size_t shift_bytes = 0) {
   return kCompactSlots * ((size_t)T.PYZ * sizeof(C) + (size_t)T.RYZ * 4) + (size_t)(T.TYi + 4 + T.TZi + 4) * 4 +
-         2 * (size_t)T.RYZ * 4;
+         2 * (size_t)T.RYZ * 4 + kCompactSlots * (size_t)T.RYZ * shift_bytes;
 }
 
 // Work lists of the compact kernel: for every tile and plane, the psi-region
@@ -1780,17 +1966,27 @@ static mplbm_status build_compact(mplbm_ctx* ctx) {
   if (3ull * kQ * ps * (uint64_t)ctx->bytes_pop >= (1ull << 32))
     return fail(ctx, MPLBM_ERR_ARG, "planes too large for the compact layout");
   const size_t lat_bytes = (size_t)(lx + 2) * kQ * ps * ctx->bytes_pop;
+  const size_t shift_bytes = ctx->cfg.shift == MPLBM_SHIFT_NODE ? (size_t)lx * ps * ctx->bytes_pop : 0;
   for (int k = 0; k < 2; ++k) {
     if (ctx->lat[k]) HIP_TRY(ctx, hipFree(ctx->lat[k]));
     ctx->lat[k] = nullptr;
+    if (ctx->shift[k]) HIP_TRY(ctx, hipFree(ctx->shift[k]));
+    ctx->shift[k] = nullptr;
   }
   if (ctx->rw) HIP_TRY(ctx, hipFree(ctx->rw));
   ctx->rw = nullptr;
-  ctx->device_bytes -= (int64_t)(2 * ctx->lat_bytes);
+  ctx->device_bytes -= (int64_t)(2 * ctx->lat_bytes + 2 * ctx->shift_bytes);
   HIP_TRY(ctx, hipMalloc(&ctx->lat[0], lat_bytes));
   HIP_TRY(ctx, hipMalloc(&ctx->lat[1], lat_bytes));
+  if (shift_bytes) {
+    for (int k = 0; k < 2; ++k) {
+      HIP_TRY(ctx, hipMalloc(&ctx->shift[k], shift_bytes));
+      HIP_TRY(ctx, hipMemset(ctx->shift[k], 0, shift_bytes));
+    }
+  }
   ctx->lat_bytes = lat_bytes;
-  ctx->device_bytes += (int64_t)(2 * lat_bytes);
+  ctx->shift_bytes = shift_bytes;
+  ctx->device_bytes += (int64_t)(2 * lat_bytes + 2 * shift_bytes);
   HIP_TRY(ctx, hipMalloc((void**)&ctx->rw, rw.size() * sizeof(uint2)));
   HIP_TRY(ctx, hipMemcpy(ctx->rw, rw.data(), rw.size() * sizeof(uint2), hipMemcpyHostToDevice));
   ctx->W = W;
@@ -1806,22 +2002,24 @@ static mplbm_status launch_step(mplbm_ctx* ctx, int64_t step, int32_t reverse, u
   if (ctx->kernel == MPLBM_KERNEL_COMPACT) {
     const CTiling& T = ctx->ctiling;
     const CLists WL{ctx->cpos, ctx->ctab};
-    const size_t shm = compact_lds_bytes<C>(T);
+    const size_t shm = compact_lds_bytes<C>(T, ph.node_shift ? sizeof(typename Mo::S) : 0);
     // the workgroup size is at least the tile's interior count (one kept
     // interior node per thread); 1024 only for non-FP64 storage
-    // stochastic rounding is a separate instantiation so the round-to-nearest
-    // kernels keep the fused multiply-add-and-convert of the encode
-    auto launch_as = [&](auto sr, const CTiling& R, bool faces) {
+    // stochastic rounding and node shifts are separate instantiations so the
+    // round-to-nearest kernels keep the fused multiply-add-and-convert of the
+    // encode
+    auto launch_as = [&](auto sr, auto ns, const CTiling& R, bool faces) {
       constexpr bool kSR = decltype(sr)::value;
+      constexpr bool kNS = decltype(ns)::value;
       const int grid = R.xcd_remap ? ((R.nblocks + 7) / 8) * 8 : R.nblocks;
       using KernelPtr = void (*)(Lattice<Mo>, Phys<C>, CTiling, CLists);
       KernelPtr k = nullptr;
       if (R.threads == 256) {
-        k = faces ? k_compact<Mo, true, 256, kSR> : k_compact<Mo, false, 256, kSR>;
+        k = faces ? k_compact<Mo, true, 256, kSR, kNS> : k_compact<Mo, false, 256, kSR, kNS>;
       } else if (R.threads == 512) {
-        k = faces ? k_compact<Mo, true, 512, kSR> : k_compact<Mo, false, 512, kSR>;
+        k = faces ? k_compact<Mo, true, 512, kSR, kNS> : k_compact<Mo, false, 512, kSR, kNS>;
       } else if constexpr (Mo::kStorage != MPLBM_FP64) {
-        k = faces ? k_compact<Mo, true, 1024, kSR> : k_compact<Mo, false, 1024, kSR>;
+        k = faces ? k_compact<Mo, true, 1024, kSR, kNS> : k_compact<Mo, false, 1024, kSR, kNS>;
       }
       if (!k) return;
       // dynamic LDS above 64 KiB (gfx950 gives a workgroup up to 160 KiB)
@@ -1831,11 +2029,15 @@ static mplbm_status launch_step(mplbm_ctx* ctx, int64_t step, int32_t reverse, u
     auto launch = [&](const CTiling& R, bool faces) {
       if constexpr (Mo::kDev) {
         if (ph.sr) {
-          launch_as(std::true_type{}, R, faces);
+          launch_as(std::true_type{}, std::false_type{}, R, faces);
+          return;
+        }
+        if (ph.node_shift) {
+          launch_as(std::false_type{}, std::true_type{}, R, faces);
           return;
         }
       }
-      launch_as(std::false_type{}, R, faces);
+      launch_as(std::false_type{}, std::false_type{}, R, faces);
     };
     const bool x_faces_only = ph.any_face && !(ph.face[2] | ph.face[3] | ph.face[4] | ph.face[5]);
     if (x_faces_only && T.nseg >= 3 && T.seg >= 2) {
@@ -1952,6 +2154,14 @@ mplbm_status mplbm_create(const mplbm_config* cfg, mplbm_ctx** out) {
        (cfg->kernel != MPLBM_KERNEL_AUTO && cfg->kernel != MPLBM_KERNEL_COMPACT)))
     return fail(nullptr, MPLBM_ERR_UNSUPPORTED,
                 "stochastic rounding applies to FP16S/BF16S storage with the compact kernel");
+  if (cfg->shift != MPLBM_SHIFT_GLOBAL && cfg->shift != MPLBM_SHIFT_NODE)
+    return fail(nullptr, MPLBM_ERR_ARG, "bad shift");
+  if (cfg->shift == MPLBM_SHIFT_NODE &&
+      ((cfg->storage != MPLBM_FP16S && cfg->storage != MPLBM_BF16S) ||
+       (cfg->kernel != MPLBM_KERNEL_AUTO && cfg->kernel != MPLBM_KERNEL_COMPACT) ||
+       cfg->rounding != MPLBM_ROUND_NEAREST))
+    return fail(nullptr, MPLBM_ERR_UNSUPPORTED,
+                "node shifts apply to FP16S/BF16S storage with the compact kernel and round-to-nearest");
 
   mplbm_ctx* ctx = new mplbm_ctx();
   ctx->cfg = *cfg;
@@ -2017,7 +2227,7 @@ mplbm_status mplbm_create(const mplbm_config* cfg, mplbm_ctx** out) {
 void mplbm_destroy(mplbm_ctx* ctx) {
   if (!ctx) return;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  void* bufs[] = {ctx->lat[0], ctx->lat[1], ctx->link,     ctx->mask,      ctx->rho,      ctx->pres,
+  void* bufs[] = {ctx->lat[0], ctx->lat[1], ctx->shift[0], ctx->shift[1], ctx->link, ctx->mask, ctx->rho, ctx->pres,
                   ctx->u,      ctx->psi,    ctx->red_partial, ctx->red_out, ctx->rw,       ctx->cpos,
                   ctx->ctab,   ctx->cl_parent, ctx->cl_size, ctx->cl_phase, ctx->cl_acc, ctx->cl_sid};
   for (void* p : bufs)
@@ -2098,7 +2308,8 @@ mplbm_status mplbm_init_fields_f64(mplbm_ctx* ctx, const double* rho) {
       const auto ph = make_phys<typename Mo::C>(ctx, 0, 1, 0, false);
       auto L = make_lattice<Mo>(ctx, 0, 0);
       hipLaunchKernelGGL(k_init<Mo>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, L,
-                         static_cast<typename Mo::S*>(ctx->lat[1]), ph, d_rho);
+                         static_cast<typename Mo::S*>(ctx->lat[1]), static_cast<typename Mo::S*>(ctx->shift[1]), ph,
+                         d_rho);
       if (!ctx->compact)
         hipLaunchKernelGGL(k_ghost<Mo>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, L);
       return MPLBM_OK;
@@ -2268,8 +2479,9 @@ struct CheckpointHeader {
   uint64_t mask_hash;     // FNV-1a of the fluid mask
   double rho_ref, dev_scale;
   uint32_t bytes_per_pop;
-  uint32_t reserved;
+  uint32_t flags;         // kCheckpointNodeShift: node shifts follow the populations
 };
+constexpr uint32_t kCheckpointNodeShift = 1u;
 
 static uint64_t mask_hash(const std::vector<uint8_t>& m) {
   uint64_t h = 1469598103934665603ull;
@@ -2284,10 +2496,12 @@ struct FileCloser {
   }
 };
 
+// Plane by plane: the 19 stored values of every cell ([y][z][19]), or with
+// shifts = true the node shifts ([y][z]), to or from host buffers.
 template <class Fn>
-static mplbm_status plane_raw_pass(mplbm_ctx* ctx, bool scatter, Fn&& per_plane) {
+static mplbm_status plane_raw_pass(mplbm_ctx* ctx, bool scatter, Fn&& per_plane, bool shifts = false) {
   const uint64_t plane = ctx->cfg.ly * ctx->cfg.lz;
-  const size_t bytes = plane * kQ * (size_t)ctx->bytes_pop;
+  const size_t bytes = plane * (shifts ? 1 : kQ) * (size_t)ctx->bytes_pop;
   void* d = nullptr;
   HIP_TRY(ctx, hipMalloc(&d, bytes));
   std::vector<unsigned char> h(bytes);
@@ -2304,8 +2518,12 @@ static mplbm_status plane_raw_pass(mplbm_ctx* ctx, bool scatter, Fn&& per_plane)
     st = dispatch(ctx, [&](auto mo) -> mplbm_status {
       using Mo = decltype(mo);
       auto L = make_lattice<Mo>(ctx, ctx->cur, ctx->cur);
-      hipLaunchKernelGGL(k_plane_raw<Mo>, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, ctx->stream, L,
-                         (int)x, reinterpret_cast<typename Mo::S*>(d), scatter ? 1 : 0);
+      if (shifts)
+        hipLaunchKernelGGL(k_plane_shift<Mo>, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, ctx->stream, L,
+                           (int)x, reinterpret_cast<typename Mo::S*>(d), scatter ? 1 : 0);
+      else
+        hipLaunchKernelGGL(k_plane_raw<Mo>, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, ctx->stream, L,
+                           (int)x, reinterpret_cast<typename Mo::S*>(d), scatter ? 1 : 0);
       return MPLBM_OK;
     });
     if (st != MPLBM_OK) break;
@@ -2349,15 +2567,26 @@ mplbm_status mplbm_save_checkpoint(mplbm_ctx* ctx, const char* path, int64_t ste
   hd.rho_ref = ctx->rho_ref;
   hd.dev_scale = ctx->scale;
   hd.bytes_per_pop = (uint32_t)ctx->bytes_pop;
+  const bool node_shift = ctx->cfg.shift == MPLBM_SHIFT_NODE;
+  hd.flags = node_shift ? kCheckpointNodeShift : 0u;
   if (fwrite(&hd, sizeof hd, 1, fc.f) != 1) return fail(ctx, MPLBM_ERR_ARG, "write error on %s", path);
   const uint64_t plane = ctx->cfg.ly * ctx->cfg.lz;
   const size_t rec = kQ * (size_t)ctx->bytes_pop;
-  return plane_raw_pass(ctx, false, [&](int64_t x, unsigned char* h) -> mplbm_status {
+  mplbm_status st = plane_raw_pass(ctx, false, [&](int64_t x, unsigned char* h) -> mplbm_status {
     const uint8_t* m = ctx->hmask.data() + (size_t)x * plane;
     for (uint64_t r = 0; r < plane; ++r)
       if (m[r] && fwrite(h + r * rec, rec, 1, fc.f) != 1) return fail(ctx, MPLBM_ERR_ARG, "write error on %s", path);
     return MPLBM_OK;
   });
+  if (st != MPLBM_OK || !node_shift) return st;
+  // node-shifted form: then the shift of every fluid node, plane by plane
+  const size_t srec = (size_t)ctx->bytes_pop;
+  return plane_raw_pass(ctx, false, [&](int64_t x, unsigned char* h) -> mplbm_status {
+    const uint8_t* m = ctx->hmask.data() + (size_t)x * plane;
+    for (uint64_t r = 0; r < plane; ++r)
+      if (m[r] && fwrite(h + r * srec, srec, 1, fc.f) != 1) return fail(ctx, MPLBM_ERR_ARG, "write error on %s", path);
+    return MPLBM_OK;
+  }, true);
 }
 
 mplbm_status mplbm_load_checkpoint(mplbm_ctx* ctx, const char* path, int64_t* step) {
@@ -2375,6 +2604,10 @@ mplbm_status mplbm_load_checkpoint(mplbm_ctx* ctx, const char* path, int64_t* st
     return fail(ctx, MPLBM_ERR_ARG, "checkpoint storage %u differs from the context's", hd.storage);
   if (hd.fluid_nodes != ctx->fluid_nodes || hd.mask_hash != mask_hash(ctx->hmask))
     return fail(ctx, MPLBM_ERR_ARG, "checkpoint geometry differs from the context's");
+  const bool node_shift = ctx->cfg.shift == MPLBM_SHIFT_NODE;
+  if (((hd.flags & kCheckpointNodeShift) != 0) != node_shift)
+    return fail(ctx, MPLBM_ERR_ARG, "checkpoint shift form (%s) differs from the context's",
+                (hd.flags & kCheckpointNodeShift) ? "node" : "global");
   // the stored deviations are relative to the file's reference density and scale
   ctx->rho_ref = (float)hd.rho_ref;
   ctx->scale = (float)hd.dev_scale;
@@ -2387,6 +2620,17 @@ mplbm_status mplbm_load_checkpoint(mplbm_ctx* ctx, const char* path, int64_t* st
     return MPLBM_OK;
   });
   if (st != MPLBM_OK) return st;
+  if (node_shift) {
+    const size_t srec = (size_t)ctx->bytes_pop;
+    st = plane_raw_pass(ctx, true, [&](int64_t x, unsigned char* h) -> mplbm_status {
+      const uint8_t* m = ctx->hmask.data() + (size_t)x * plane;
+      std::memset(h, 0, plane * srec);
+      for (uint64_t r = 0; r < plane; ++r)
+        if (m[r] && fread(h + r * srec, srec, 1, fc.f) != 1) return fail(ctx, MPLBM_ERR_ARG, "%s is truncated", path);
+      return MPLBM_OK;
+    }, true);
+    if (st != MPLBM_OK) return st;
+  }
   if (!ctx->compact) {  // bounce-back slots of the full layout
     st = dispatch(ctx, [&](auto mo) -> mplbm_status {
       using Mo = decltype(mo);
@@ -2905,6 +3149,9 @@ mplbm_status mplbm_ensemble_create(const mplbm_config* cfgs, int32_t n, int32_t 
   for (int32_t s = 0; s < n; ++s)
     if (cfgs[s].rounding != MPLBM_ROUND_NEAREST)  // a replayed graph would repeat the dither of one step
       return ens_fail(nullptr, MPLBM_ERR_UNSUPPORTED, "the ensemble stores with round-to-nearest only");
+  for (int32_t s = 0; s < n; ++s)
+    if (cfgs[s].shift != MPLBM_SHIFT_GLOBAL)
+      return ens_fail(nullptr, MPLBM_ERR_UNSUPPORTED, "the ensemble stores with the global shift only");
   const bool compact = cfgs[0].kernel != MPLBM_KERNEL_FUSED;
   if (compact && cfgs[0].storage == MPLBM_FP64 && block_threads > 512)
     return ens_fail(nullptr, MPLBM_ERR_ARG, "FP64 compact ensemble needs block_threads <= 512");
