@@ -42,8 +42,9 @@ def parse():
     ap.add_argument("--rmin", type=float, default=6.0)
     ap.add_argument("--rmax", type=float, default=18.0)
     ap.add_argument("--seed", type=int, default=3)
-    ap.add_argument("--storage", default="fp32,fp32fast,fp16s,bf16s,fp16sr",
-                    help="fp32, fp32fast, fp16s, bf16s, fp64; fp16sr/bf16sr: 16-bit with stochastic rounding")
+    ap.add_argument("--storage", default="fp32,fp32fast,fp16s,bf16s,fp16sr,fp16sn",
+                    help="fp32, fp32fast, fp16s, bf16s, fp64; fp16sr/bf16sr: 16-bit with stochastic rounding; "
+                         "fp16sn/bf16sn: 16-bit with node shifts")
     ap.add_argument("--kernel", default="compact", help="compact (fluid nodes only), fused or twopass")
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--tile-z", type=int, default=0)
@@ -91,7 +92,15 @@ def workload(a):
 
 PHYS = dict(tau=1.0, G=-5.0, rhos=0.7, rhol_spinodal=1.285931, rhog_spinodal=0.323507)
 # storage names of bench.py and the stored form they time
-BASE_STORAGE = {"fp32fast": "fp32", "fp16sr": "fp16s", "bf16sr": "bf16s"}
+BASE_STORAGE = {"fp32fast": "fp32", "fp16sr": "fp16s", "bf16sr": "bf16s", "fp16sn": "fp16s", "bf16sn": "bf16s"}
+
+
+def bytes_per_update(mp, storage):
+    """Algorithmic bytes of one fluid-node update: 19 populations read and
+    written, plus the node's 16-bit shift read and written in the node-shifted
+    forms."""
+    b = 2 * Q * mp.STORAGE_BYTES[mp.STORAGE_NAMES[BASE_STORAGE.get(storage, storage)]]
+    return b + (4 if storage in ("fp16sn", "bf16sn") else 0)
 
 
 def run_gpu(a, shape, fluid, rho, storage):
@@ -119,15 +128,15 @@ def run_gpu(a, shape, fluid, rho, storage):
     nf = info["fluid_nodes"]
     launches = a.steps * (2 if a.kernel == "twopass" else 1)
     per_launch_s = ev_ms / 1e3 / launches
-    bpop = mp.STORAGE_BYTES[mp.STORAGE_NAMES[BASE_STORAGE.get(storage, storage)]]
-    alg_bytes = 2 * Q * bpop * nf  # per step
+    bpu = bytes_per_update(mp, storage)
+    alg_bytes = bpu * nf  # per step
     return {
         "mlups": nf * a.steps / wall / 1e6,
         "mlups_event": nf * a.steps / (ev_ms / 1e3) / 1e6,
         "ms_per_step": wall * 1e3 / a.steps,
         "kernel_ms": per_launch_s * 1e3,
         "achieved_gbs": alg_bytes / (ev_ms / 1e3 / a.steps) / 1e9,
-        "bytes_per_update": 2 * Q * bpop,
+        "bytes_per_update": bpu,
         "fluid_nodes": nf,
         "tile": [info["tile_y"], info["tile_z"], info["seg_x"]],
         "tag": f"{a.kernel}:{info['tile_y']}x{info['tile_z']}x{info['seg_x']}:bb{a.bounce_back}",
@@ -204,11 +213,12 @@ def precision_study(a, shape, fluid, rho):
     sat_thr = 0.5 * (1.932442 + 0.156413)
     two = {"steps": steps, "fp64": {"saturation": float((ref_r[fl] > sat_thr).mean()),
                                     "mass_rel_drift": ref_st["mass"] / float(rho.astype(np.float64)[fl].sum()) - 1}}
-    variants = [("fp32", {}), ("fp16s", {}), ("fp16s", {"rounding": "stochastic"}), ("bf16s", {}),
-                ("bf16s", {"rounding": "stochastic"})]
+    variants = [("fp32", {}), ("fp16s", {}), ("fp16s", {"rounding": "stochastic"}), ("fp16s", {"shift": "node"}),
+                ("bf16s", {}), ("bf16s", {"rounding": "stochastic"}), ("bf16s", {"shift": "node"})]
 
     def vkey(storage, kw):
-        return storage + ("_stochastic" if kw.get("rounding") == "stochastic" else "")
+        return (storage + ("_stochastic" if kw.get("rounding") == "stochastic" else "")
+                + ("_node" if kw.get("shift") == "node" else ""))
 
     for storage, kw in variants:
         r, _, st, dt, info, r_early = run(storage, PHYS, steps, rho, early, **kw)
@@ -314,8 +324,8 @@ def ensemble_bench(a):
                        f"G~U[-5.5,-4.5], seeds 0..{len(samples) - 1}",
            "cells": cells, "fluid_nodes": nf}
     for storage in a.storage.split(","):
-        if storage in ("fp16sr", "bf16sr"):
-            continue  # the ensemble form stores with round-to-nearest only
+        if storage in ("fp16sr", "bf16sr", "fp16sn", "bf16sn"):
+            continue  # the ensemble form stores with round-to-nearest and the global shift only
         def make(block):
             e = Ensemble(samples, storage=BASE_STORAGE.get(storage, storage), block_threads=block, kernel=a.kernel,
                          math="fast" if storage == "fp32fast" else "reference")
@@ -388,7 +398,7 @@ def drainage_bench(a):
             ctx.step(0, a.warmup)
             ms = ctx.step_timed(a.warmup, a.steps) / a.steps
             st = ctx.reduce()
-        bpu = 2 * Q * mp.STORAGE_BYTES[mp.STORAGE_NAMES[BASE_STORAGE.get(storage, storage)]]
+        bpu = bytes_per_update(mp, storage)
         out[storage] = {"ms_per_step": round(ms, 4), "mlups": round(nf / ms / 1e3, 1),
                         "roofline_frac": round(nf * bpu / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
                         "fluid_nodes": nf, "nonfinite": st["nonfinite"]}
@@ -398,7 +408,7 @@ def drainage_bench(a):
     sfluid, srho = drainage_fields(sub, 6, 0.4)
     sfl = sfluid.astype(bool)
     res = {}
-    for storage in ("fp64", "fp32", "fp16s", "fp16sr"):
+    for storage in ("fp64", "fp32", "fp16s", "fp16sr", "fp16sn"):
         t = time.time()
         with mp.Context(*sub, storage=storage, kernel=a.kernel, **DRAIN) as ctx:
             ctx.set_geometry(sfluid)
@@ -415,7 +425,7 @@ def drainage_bench(a):
         res[storage] = (r, st, time.time() - t)
     ref = res["fp64"][0]
     sv = {"shape": list(sub), "steps": a.drainage_steps, "fp64": {"saturation": float((ref[sfl] > thr).mean())}}
-    for storage in ("fp32", "fp16s", "fp16sr"):
+    for storage in ("fp32", "fp16s", "fp16sr", "fp16sn"):
         r, st, dt = res[storage]
         sv[storage] = {"saturation": float((r[sfl] > thr).mean()),
                        "saturation_abs_err": abs(float((r[sfl] > thr).mean()) - sv["fp64"]["saturation"]),

@@ -58,6 +58,7 @@ typedef struct mplbm_input {
   int32_t rounding;         /* mplbm_rounding (token rounding nearest|stochastic) */
   int32_t device_voxeliser; /* 1: read_positions and generate_solids voxelise on the
                                device (mplbm_place_spheres); 0: on the host */
+  int32_t shift;            /* mplbm_shift (token shift global|node) */
 } mplbm_input;
 
 /* Statistics of MultiphaseLBM::calc_cluster (src/multiphase_lbm.cu:225-562). */

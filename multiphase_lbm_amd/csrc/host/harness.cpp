@@ -651,6 +651,7 @@ struct MultiphaseLBM {
     cfg.kernel = in.kernel;
     cfg.math = in.math;
     cfg.rounding = in.rounding;
+    cfg.shift = in.shift;
     cfg.tau = (float)in.tau;
     cfg.G = (float)in.G;
     cfg.carn_star = in.carn_star;
@@ -899,6 +900,10 @@ bool MultiphaseLBM::mplbm_parse_scalar_token(std::istream& file, const std::stri
   } else if (token == "body_force") file >> in.body_force[0] >> in.body_force[1] >> in.body_force[2];
   else if (token == "host_statistics") file >> in.host_statistics;
   else if (token == "device_voxeliser") file >> in.device_voxeliser;
+  else if (token == "shift") {
+    file >> s;
+    in.shift = s == "node" ? MPLBM_SHIFT_NODE : MPLBM_SHIFT_GLOBAL;
+  }
   else if (token == "checkpointPeriod") file >> in.checkpointPeriod;
   else if (token == "rounding") {
     file >> s;

@@ -40,7 +40,7 @@ class Input(ctypes.Structure):
         ("body_force", ctypes.c_double * 3), ("unknown_tokens", ctypes.c_int32),
         ("host_statistics", ctypes.c_int32), ("read_checkpoint", ctypes.c_int32),
         ("checkpointPeriod", I64), ("checkpoint_filename", ctypes.c_char * 256),
-        ("rounding", ctypes.c_int32), ("device_voxeliser", ctypes.c_int32),
+        ("rounding", ctypes.c_int32), ("device_voxeliser", ctypes.c_int32), ("shift", ctypes.c_int32),
     ]
 
     def as_dict(self):

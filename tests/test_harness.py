@@ -44,7 +44,7 @@ def test_parse_extension_tokens(tmp_path):
     p = tmp_path / "in.txt"
     p.write_text("lx 32 ly 16 lz 8 # sizes\nstorage fp16s\nkernel twopass\npressure_x 2.0 0.15\n"
                  "body_force 1e-6 0 0\nbogus_token\ncheckpointPeriod 500\nread_checkpoint run/ck.mplbm\n"
-                 "host_statistics 1\nrounding stochastic\n")
+                 "host_statistics 1\nrounding stochastic\nshift node\n")
     d = H.parse_input(p)
     assert (d["lx"], d["ly"], d["lz"]) == (32, 16, 8)
     assert d["storage"] == 2 and d["kernel"] == 2
@@ -53,7 +53,7 @@ def test_parse_extension_tokens(tmp_path):
     assert d["unknown_tokens"] == 1
     assert d["checkpointPeriod"] == 500 and d["read_checkpoint"] == 1
     assert d["checkpoint_filename"] == "run/ck.mplbm" and d["host_statistics"] == 1
-    assert d["rounding"] == 1
+    assert d["rounding"] == 1 and d["shift"] == 1
 
 
 def test_place_sphere_integer_centre_is_the_open_ball():
