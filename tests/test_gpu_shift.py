@@ -58,12 +58,17 @@ def test_node_shift_tracks_fp64_closer_than_global_shift(storage):
     assert abs(drift_n) < abs(drift_g)
 
 
-def test_node_shift_with_pressure_faces():
-    # x faces (anti-bounce-back from the node's own populations) against FP32
+@pytest.mark.parametrize("axis", [0, 1])
+def test_node_shift_with_pressure_faces(axis):
+    # faces on x (face kernel on the end segments only) or y (face kernel
+    # everywhere): anti-bounce-back from the node's own populations, against FP32
     shape = (24, 18, 32)
     fluid, rho = _case(shape, 5)
-    params = dict(SC, G=-3.0, face_type=(1, 1, 0, 0, 0, 0), face_rho=(1.04, 0.96, 0, 0, 0, 0))
-    print("case node shift faces", flush=True)
+    ft, fr = [0] * 6, [0.0] * 6
+    ft[2 * axis] = ft[2 * axis + 1] = 1
+    fr[2 * axis], fr[2 * axis + 1] = 1.04, 0.96
+    params = dict(SC, G=-3.0, face_type=tuple(ft), face_rho=tuple(fr))
+    print(f"case node shift faces axis {axis}", flush=True)
     r32, _, _ = _run(shape, fluid, rho, "fp32", 300, **params)
     rn, _, bad = _run(shape, fluid, rho, "fp16s", 300, shift="node", **params)
     rg, _, _ = _run(shape, fluid, rho, "fp16s", 300, **params)
@@ -100,6 +105,24 @@ def test_node_shift_restart_is_bit_identical(tmp_path, storage):
         c.set_geometry(fluid)
         with pytest.raises(mp.MplbmError, match="shift"):
             c.load_checkpoint(path)
+
+
+@pytest.mark.parametrize("tiles", [(8, 32, 5, 0), (32, 32, 32, 1024), (3, 5, 2, 256)])
+def test_node_shift_results_do_not_depend_on_the_tiling(tiles):
+    # workgroups of 256, 512 and 1024 threads, partial tiles and short
+    # segments give the default tiling's populations bit for bit
+    shape = (12, 40, 64)
+    fluid, rho = _case(shape, 13)
+    ty, tz, seg, nt = tiles
+    print(f"case node shift tiling {tiles}", flush=True)
+    out = []
+    for kw in ({}, dict(tile_y=ty, tile_z=tz, seg_x=seg, block_threads=nt)):
+        with mp.Context(*shape, storage="fp16s", kernel="compact", shift="node", **SC, **kw) as ctx:
+            ctx.set_geometry(fluid)
+            ctx.init_fields(rho.astype(np.float32))
+            ctx.step(0, 25)
+            out.append(ctx.read_populations())
+    assert np.array_equal(out[0][fluid], out[1][fluid])
 
 
 def test_node_shift_population_round_trip():
