@@ -1850,7 +1850,13 @@ static mplbm_status choose_ctiling(mplbm_ctx* ctx) {
   const int64_t nb = tiles * T.nseg;
   if (nb > (1LL << 30)) return fail(ctx, MPLBM_ERR_ARG, "too many blocks");
   T.nblocks = (int)nb;
-  T.xcd_remap = 1;
+  // plain dispatch order: the hardware deals consecutive workgroups (the
+  // tiles of one segment) to the eight XCDs in turn, so every XCD works on
+  // the same planes at once and their ring re-reads meet in the device-wide
+  // last-level cache; giving each XCD a contiguous range of tiles (its own
+  // segments) measured 2.7% slower at 512^3 (3.60 against 3.50 ms) and 0.8%
+  // at 1024^3
+  T.xcd_remap = 0;
   T.seg0 = 0;
   T.seg_stride = 1;
   if (c.block_threads != 0 && c.block_threads != 256 && c.block_threads != 512 && c.block_threads != 1024)
