@@ -1794,7 +1794,7 @@ static mplbm_status choose_tiling(mplbm_ctx* ctx) {
   const int64_t nb = (int64_t)T.tiles_y * T.tiles_z * T.nseg;
   if (nb > (1LL << 30)) return fail(ctx, MPLBM_ERR_ARG, "too many blocks");
   T.nblocks = (int)nb;
-  T.xcd_remap = 1;
+  T.xcd_remap = 0;  // hardware dispatch order, as the compact kernel (6.29 -> 6.09 ms at 512^3)
   T.pull_bb = c.bounce_back == MPLBM_BB_PULL ? 1 : 0;
   ctx->tiling = T;
   return MPLBM_OK;
