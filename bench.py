@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--storage", default="fp32,fp32fast,fp16s,bf16s,fp16sr,fp16sn",
                     help="fp32, fp32fast, fp16s, bf16s, fp64; fp16sr/bf16sr: 16-bit with stochastic rounding; "
-                         "fp16sn/bf16sn: 16-bit with node shifts")
+                         "fp16sn/bf16sn: 16-bit with node shifts; fp16srn/bf16srn: both")
     ap.add_argument("--kernel", default="compact", help="compact (fluid nodes only), fused or twopass")
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--tile-z", type=int, default=0)
@@ -92,7 +92,8 @@ def workload(a):
 
 PHYS = dict(tau=1.0, G=-5.0, rhos=0.7, rhol_spinodal=1.285931, rhog_spinodal=0.323507)
 # storage names of bench.py and the stored form they time
-BASE_STORAGE = {"fp32fast": "fp32", "fp16sr": "fp16s", "bf16sr": "bf16s", "fp16sn": "fp16s", "bf16sn": "bf16s"}
+BASE_STORAGE = {"fp32fast": "fp32", "fp16sr": "fp16s", "bf16sr": "bf16s", "fp16sn": "fp16s", "bf16sn": "bf16s",
+                "fp16srn": "fp16s", "bf16srn": "bf16s"}
 
 
 def bytes_per_update(mp, storage):
@@ -100,7 +101,7 @@ def bytes_per_update(mp, storage):
     written, plus the node's 16-bit shift read and written in the node-shifted
     forms."""
     b = 2 * Q * mp.STORAGE_BYTES[mp.STORAGE_NAMES[BASE_STORAGE.get(storage, storage)]]
-    return b + (4 if storage in ("fp16sn", "bf16sn") else 0)
+    return b + (4 if storage in ("fp16sn", "bf16sn", "fp16srn", "bf16srn") else 0)
 
 
 def run_gpu(a, shape, fluid, rho, storage):
@@ -214,7 +215,9 @@ def precision_study(a, shape, fluid, rho):
     two = {"steps": steps, "fp64": {"saturation": float((ref_r[fl] > sat_thr).mean()),
                                     "mass_rel_drift": ref_st["mass"] / float(rho.astype(np.float64)[fl].sum()) - 1}}
     variants = [("fp32", {}), ("fp16s", {}), ("fp16s", {"rounding": "stochastic"}), ("fp16s", {"shift": "node"}),
-                ("bf16s", {}), ("bf16s", {"rounding": "stochastic"}), ("bf16s", {"shift": "node"})]
+                ("fp16s", {"rounding": "stochastic", "shift": "node"}),
+                ("bf16s", {}), ("bf16s", {"rounding": "stochastic"}), ("bf16s", {"shift": "node"}),
+                ("bf16s", {"rounding": "stochastic", "shift": "node"})]
 
     def vkey(storage, kw):
         return (storage + ("_stochastic" if kw.get("rounding") == "stochastic" else "")
@@ -324,7 +327,7 @@ def ensemble_bench(a):
                        f"G~U[-5.5,-4.5], seeds 0..{len(samples) - 1}",
            "cells": cells, "fluid_nodes": nf}
     for storage in a.storage.split(","):
-        if storage in ("fp16sr", "bf16sr", "fp16sn", "bf16sn"):
+        if storage in ("fp16sr", "bf16sr", "fp16sn", "bf16sn", "fp16srn", "bf16srn"):
             continue  # the ensemble form stores with round-to-nearest and the global shift only
         def make(block):
             e = Ensemble(samples, storage=BASE_STORAGE.get(storage, storage), block_threads=block, kernel=a.kernel,
@@ -408,7 +411,7 @@ def drainage_bench(a):
     sfluid, srho = drainage_fields(sub, 6, 0.4)
     sfl = sfluid.astype(bool)
     res = {}
-    for storage in ("fp64", "fp32", "fp16s", "fp16sr", "fp16sn"):
+    for storage in ("fp64", "fp32", "fp16s", "fp16sr", "fp16sn", "fp16srn"):
         t = time.time()
         with mp.Context(*sub, storage=storage, kernel=a.kernel, **DRAIN) as ctx:
             ctx.set_geometry(sfluid)
@@ -425,7 +428,7 @@ def drainage_bench(a):
         res[storage] = (r, st, time.time() - t)
     ref = res["fp64"][0]
     sv = {"shape": list(sub), "steps": a.drainage_steps, "fp64": {"saturation": float((ref[sfl] > thr).mean())}}
-    for storage in ("fp32", "fp16s", "fp16sr", "fp16sn"):
+    for storage in ("fp32", "fp16s", "fp16sr", "fp16sn", "fp16srn"):
         r, st, dt = res[storage]
         sv[storage] = {"saturation": float((r[sfl] > thr).mean()),
                        "saturation_abs_err": abs(float((r[sfl] > thr).mean()) - sv["fp64"]["saturation"]),

@@ -98,7 +98,7 @@ typedef enum mplbm_rounding { MPLBM_ROUND_NEAREST = 0, MPLBM_ROUND_STOCHASTIC = 
  * from that node's density (rounded to the 16-bit type and stored beside
  * them, 2 B per node), g_i = (f_i/w_i - rho_node)*scale, so the stored
  * values stay small in both phases of a two-phase run; 4 more bytes per
- * update, round-to-nearest only. */
+ * update; combines with either rounding. */
 typedef enum mplbm_shift { MPLBM_SHIFT_GLOBAL = 0, MPLBM_SHIFT_NODE = 1 } mplbm_shift;
 
 /* All fields of transfer_parameter (src/iteration_host.cu:16-41) plus the

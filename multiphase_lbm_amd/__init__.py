@@ -65,7 +65,8 @@ class Context:
     inject_position, face_type, face_rho, body_force, rho_ref, dev_scale,
     tile_y, tile_z, seg_x, math, rounding, shift).  storage "fp32fast" is FP32
     storage with math="fast"; "fp16sr"/"bf16sr" add rounding="stochastic" and
-    "fp16sn"/"bf16sn" shift="node" to the 16-bit forms."""
+    "fp16sn"/"bf16sn" shift="node", "fp16srn"/"bf16srn" both, to the 16-bit
+    forms."""
 
     def __init__(self, lx, ly, lz, storage="fp32", kernel="auto", **params):
         self.lib = _abi.load()
@@ -77,6 +78,8 @@ class Context:
             storage, params = storage[:-1], dict(params, rounding="stochastic")
         elif storage in ("fp16sn", "bf16sn"):
             storage, params = storage[:-1], dict(params, shift="node")
+        elif storage in ("fp16srn", "bf16srn"):
+            storage, params = storage[:-2], dict(params, rounding="stochastic", shift="node")
         cfg.storage = STORAGE_NAMES[storage] if isinstance(storage, str) else int(storage)
         cfg.kernel = KERNEL_NAMES[kernel] if isinstance(kernel, str) else int(kernel)
         for k, v in params.items():

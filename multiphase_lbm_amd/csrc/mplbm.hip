@@ -195,8 +195,8 @@ __host__ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
 // with probability equal to its distance from the value below.
 template <class Mo>
 __device__ __forceinline__ typename Mo::S encode_sr(typename Mo::C v, int i, const Phys<typename Mo::C>& ph,
-                                                   uint32_t rnd) {
-  const float g = __builtin_fmaf(v, ph.enc_mul[wclass(i)], ph.enc_add);
+                                                   uint32_t rnd, float enc_add) {
+  const float g = __builtin_fmaf(v, ph.enc_mul[wclass(i)], enc_add);
   // 2^floor(log2|g|) from the exponent bits; the format's unit in the last
   // place is 2^-10 (FP16, at least 2^-24) or 2^-7 (BF16) of it
   const float p2 = __uint_as_float(__float_as_uint(g) & 0x7F800000u);
@@ -1218,10 +1218,16 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
       // dither stream of this node and step: a mixed seed, then a linear
       // congruential sequence over the populations
       uint32_t rs = mix32(own ^ ((uint32_t)xw * 0x9E3779B1u) ^ ph.sr_seed);
+      float enc_add = ph.enc_add;
+      if constexpr (kNS) {  // node shift: deviations from the node's 16-bit density
+        const S sh = shift_of<Mo>((float)rho);
+        enc_add = -shift_value<Mo>(sh) * ph.scale;
+        L.sout[sidx(L, xw, own / sz)] = sh;
+      }
 #pragma unroll
       for (int i = 0; i < kQ; ++i) {
         rs = rs * 1664525u + 1013904223u;
-        e[i] = encode_sr<Mo>(f[i], i, ph, rs);
+        e[i] = encode_sr<Mo>(f[i], i, ph, rs, enc_add);
       }
     } else if constexpr (kNS) {
       // the node's shift: its density in the 16-bit type, stored beside it
@@ -2035,7 +2041,8 @@ static mplbm_status launch_step(mplbm_ctx* ctx, int64_t step, int32_t reverse, u
     auto launch = [&](const CTiling& R, bool faces) {
       if constexpr (Mo::kDev) {
         if (ph.sr) {
-          launch_as(std::true_type{}, std::false_type{}, R, faces);
+          if (ph.node_shift) launch_as(std::true_type{}, std::true_type{}, R, faces);
+          else launch_as(std::true_type{}, std::false_type{}, R, faces);
           return;
         }
         if (ph.node_shift) {
@@ -2164,10 +2171,8 @@ mplbm_status mplbm_create(const mplbm_config* cfg, mplbm_ctx** out) {
     return fail(nullptr, MPLBM_ERR_ARG, "bad shift");
   if (cfg->shift == MPLBM_SHIFT_NODE &&
       ((cfg->storage != MPLBM_FP16S && cfg->storage != MPLBM_BF16S) ||
-       (cfg->kernel != MPLBM_KERNEL_AUTO && cfg->kernel != MPLBM_KERNEL_COMPACT) ||
-       cfg->rounding != MPLBM_ROUND_NEAREST))
-    return fail(nullptr, MPLBM_ERR_UNSUPPORTED,
-                "node shifts apply to FP16S/BF16S storage with the compact kernel and round-to-nearest");
+       (cfg->kernel != MPLBM_KERNEL_AUTO && cfg->kernel != MPLBM_KERNEL_COMPACT)))
+    return fail(nullptr, MPLBM_ERR_UNSUPPORTED, "node shifts apply to FP16S/BF16S storage with the compact kernel");
 
   mplbm_ctx* ctx = new mplbm_ctx();
   ctx->cfg = *cfg;

@@ -148,13 +148,28 @@ def test_node_shift_population_round_trip():
     assert st["nonfinite"] == 0
 
 
+def test_node_shift_with_stochastic_rounding():
+    # dithered deviations from the node's density: reproducible, different
+    # from round-to-nearest, and the mass drift stays at the node-shift level
+    # (round-to-nearest's is already ~1e-5 here, 30x below the global shift's)
+    shape = (40, 36, 64)
+    fluid, rho = _case(shape, 21)
+    print("case node shift stochastic", flush=True)
+    rn, drift_n, _ = _run(shape, fluid, rho, "fp16s", 400, shift="node", **SC)
+    rs, drift_s, bad = _run(shape, fluid, rho, "fp16s", 400, shift="node", rounding="stochastic", **SC)
+    rs2, drift_s2, _ = _run(shape, fluid, rho, "fp16s", 400, shift="node", rounding="stochastic", **SC)
+    print(f"  mass drift nearest {drift_n:.3e} stochastic {drift_s:.3e}", flush=True)
+    assert bad == 0
+    assert np.array_equal(rs[fluid], rs2[fluid]) and drift_s == drift_s2
+    assert not np.array_equal(rs[fluid], rn[fluid])
+    assert abs(drift_s) < 5e-5 and abs(drift_n) < 5e-5
+
+
 def test_node_shift_refusals():
     with pytest.raises(mp.MplbmError, match="node shifts"):
         mp.Context(8, 8, 8, storage="fp32", kernel="compact", shift="node")
     with pytest.raises(mp.MplbmError, match="node shifts"):
         mp.Context(8, 8, 8, storage="fp16s", kernel="fused", shift="node")
-    with pytest.raises(mp.MplbmError, match="node shifts"):
-        mp.Context(8, 8, 8, storage="fp16s", kernel="compact", shift="node", rounding="stochastic")
     from multiphase_lbm_amd.ensemble import Ensemble
     with pytest.raises(mp.MplbmError, match="global shift"):
         Ensemble([dict(lx=16, ly=16, lz=16, shift=mp.SHIFT_NODE)], storage="fp16s")
