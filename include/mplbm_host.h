@@ -59,6 +59,10 @@ typedef struct mplbm_input {
   int32_t device_voxeliser; /* 1: read_positions and generate_solids voxelise on the
                                device (mplbm_place_spheres); 0: on the host */
   int32_t shift;            /* mplbm_shift (token shift global|node) */
+  int32_t saturation_cycle; /* 1 (token saturation_cycle 1): cycle injection and drainage
+                               between max_saturation and min_saturation; 0: reverse stays
+                               1 as in the reference, which parses both tokens but never
+                               acts on them */
 } mplbm_input;
 
 /* Statistics of MultiphaseLBM::calc_cluster (src/multiphase_lbm.cu:225-562). */
@@ -146,6 +150,16 @@ int mplbm_host_write_vtk(const char* path, const uint8_t* is_fluid, const float*
 int mplbm_host_write_vtr(const char* path, const uint8_t* is_fluid, const float* rho, const float* pres,
                          const int64_t* liq_label, const int64_t* gas_label, const int64_t* solid_number,
                          int64_t lx, int64_t ly, int64_t lz);
+
+/* read_data (src/multiphase_lbm.cu:619-745): tokens, domain, density
+ * initialisation and geometry (resize, read_sample, place_sphere,
+ * generate_solids, read_positions), then erode_solid_touch_liquid when the
+ * input has target_density (:739-740, :842-890).  Fills `in` (lx, ly, lz =
+ * the domain); is_fluid and rho (each may be NULL, capacity cells) receive
+ * the mask and the initial density [x][y][z]; eroded (may be NULL) the
+ * number of solid nodes turned fluid.  Returns 0 or < 0. */
+int mplbm_host_read_data(const char* input_path, mplbm_input* in, uint8_t* is_fluid, float* rho, int64_t capacity,
+                         int64_t* eroded);
 
 /* The whole program of src/main.cc:4-17 (read_data, init, run_gpu) on the
  * HIP library.  Returns 0 on success. */

@@ -466,6 +466,7 @@ struct MultiphaseLBM {
   std::vector<int64_t> solid_number, liq_label, gas_label;
   int64_t solid_count = 0;
   int64_t solids = 0;
+  int64_t eroded_nodes = 0;
   float packing_density = 0.f;
   std::ofstream stat_file;
 
@@ -508,28 +509,44 @@ struct MultiphaseLBM {
     return t;
   }
 
-  void erode() {  // erode_solid_touch_liquid, :842-890
+  // erode_solid_touch_liquid, :842-890.  The eroded count is the
+  // reference's float expression (1 - target_density/packing_density) *
+  // solids, truncated.  While it exceeds the number of solids touching the
+  // fluid, whole layers of touching solids turn fluid; then each touching
+  // solid turns fluid when rand()/RAND_MAX < eroded/touching (float), in the
+  // reference's z, y, x scan order, so the draws follow its rand() stream.
+  // One deliberate difference: the touching count is recounted after every
+  // whole layer.  The reference counts it once and only decrements it; its
+  // size_t counter wraps below zero in the second layer, after which the loop
+  // either stops with two whole layers eroded and a random pass of rate ~0
+  // (target at or beyond two layers) or never ends (target inside the second
+  // layer).  A target within the first layer takes the same path in both.
+  // Returns the number of solids turned fluid.
+  int64_t erode() {
     std::vector<uint8_t> touch = solid_touch_liquid();
     int64_t solid_liquid_num = 0;
     for (uint8_t t : touch) solid_liquid_num += t;
-    if (packing_density <= 0.f || in.target_density >= packing_density) {
-      std::cerr << "target_density " << in.target_density << " is not below the packing density "
-                << packing_density << "; no erosion" << std::endl;
-      return;
+    const float target = (float)in.target_density;
+    if (packing_density <= 0.f || target >= packing_density) {
+      std::cerr << "target_density " << target << " is not below the packing density " << packing_density
+                << "; no erosion" << std::endl;
+      return 0;
     }
-    int64_t eroded = (int64_t)((1.0 - in.target_density / packing_density) * (double)solids);
+    int64_t eroded = (int64_t)((1.0f - target / packing_density) * (float)solids);
     std::cout << "Eroded solids number: " << eroded << std::endl;
     std::cout << "Solid touch liquid number: " << solid_liquid_num << std::endl;
+    int64_t done = 0;
     while (eroded > solid_liquid_num) {
       for (int64_t z = 0; z < d.lz; ++z)
         for (int64_t y = 0; y < d.ly; ++y)
           for (int64_t x = 0; x < d.lx; ++x) {
             const size_t c = d.at(x, y, z);
-            if (touch[c]) { touch[c] = 0; is_fluid[c] = 1; --eroded; --solid_liquid_num; }
+            if (touch[c]) { touch[c] = 0; is_fluid[c] = 1; --eroded; ++done; }
           }
       touch = solid_touch_liquid();
       solid_liquid_num = 0;
       for (uint8_t t : touch) solid_liquid_num += t;
+      if (solid_liquid_num == 0) break;  // no solid left next to the fluid
     }
     if (eroded <= solid_liquid_num && eroded > 0) {
       const float rate = (float)((double)eroded / (double)solid_liquid_num);
@@ -537,10 +554,11 @@ struct MultiphaseLBM {
         for (int64_t y = 0; y < d.ly; ++y)
           for (int64_t x = 0; x < d.lx; ++x) {
             const size_t c = d.at(x, y, z);
-            if (touch[c] && (float)rand() / (float)RAND_MAX < rate) { touch[c] = 0; is_fluid[c] = 1; --eroded; }
+            if (touch[c] && (float)rand() / (float)RAND_MAX < rate) { touch[c] = 0; is_fluid[c] = 1; ++done; }
           }
     }
     calc_packing_density();
+    return done;
   }
 
   int read_data(const char* path) {  // :619-745
@@ -621,7 +639,7 @@ struct MultiphaseLBM {
               << in.rho_inject_period << " step." << std::endl;
     std::cout << "Injection type: " << in.inject_type << std::endl;
     if (!allocated) return set_err("no domain: the input has neither resize nor read_sample");
-    if (in.erosion) erode();
+    eroded_nodes = in.erosion ? erode() : 0;
     return 0;
   }
 
@@ -672,6 +690,7 @@ struct MultiphaseLBM {
     };
     int rc = 0;
     int64_t step = in.step;
+    int resume_reverse = 1;
     if (!check(mplbm_set_geometry(ctx, is_fluid.data()))) {
       mplbm_destroy(ctx);
       return -1;
@@ -681,7 +700,12 @@ struct MultiphaseLBM {
         mplbm_destroy(ctx);
         return -1;
       }
-      std::cout << "Resuming from checkpoint " << in.checkpoint_filename << " at step " << step << std::endl;
+      if (read_driver_state(in.checkpoint_filename, &resume_reverse) != 0) {
+        mplbm_destroy(ctx);
+        return -1;
+      }
+      std::cout << "Resuming from checkpoint " << in.checkpoint_filename << " at step " << step
+                << (resume_reverse < 0 ? " (draining)" : "") << std::endl;
     } else if (!check(mplbm_init_fields(ctx, rho.data()))) {
       mplbm_destroy(ctx);
       return -1;
@@ -691,12 +715,14 @@ struct MultiphaseLBM {
       return -1;
     }
     std::vector<int32_t> liq32, gas32;
-    // reverse = 1 injects, -1 drains (src/multiphase_lbm.cu:571-573).  With
-    // max_saturation / min_saturation in the input (the reference parses
-    // them, :712-716, but never changes reverse) the driver cycles: inject
+    // reverse = 1 injects, -1 drains (src/multiphase_lbm.cu:571-573).  The
+    // reference parses max_saturation / min_saturation (:712-716) but never
+    // changes reverse.  With saturation_cycle 1 the driver cycles: inject
     // until the saturation reaches max_saturation, then drain until it falls
-    // to min_saturation, and so on, switching at statistics steps.
-    int reverse = 1;
+    // to min_saturation, and so on, switching at statistics steps.  The
+    // direction is driver state: it is saved beside each checkpoint and
+    // restored on resume.
+    int reverse = resume_reverse;
     const int64_t stat_period = in.statPeriod > 0 ? in.statPeriod : 1;
     const int64_t ck_period = in.checkpointPeriod;
     size_t vtk_num = 0, vtr_num = 0;
@@ -768,7 +794,7 @@ struct MultiphaseLBM {
         std::printf("saturation=%f, aLp=%f,aVp=%f, step=%ld\n", s.saturation, s.avg_liq_press, s.avg_vap_press,
                     (long)step);
         write_stats_row(step, s);
-        if (in.hysteresis) {
+        if (in.saturation_cycle) {
           if (reverse == 1 && s.saturation >= in.max_saturation) {
             reverse = -1;
             std::printf("saturation %f reached max_saturation at step %ld: draining\n", s.saturation, (long)step);
@@ -806,6 +832,7 @@ struct MultiphaseLBM {
       if (ckpt) {
         const std::string name = std::string(in.result_folder) + "/checkpoint.mplbm";
         if (!check(mplbm_save_checkpoint(ctx, name.c_str(), last + 1))) { rc = -1; break; }
+        if (write_driver_state(name, reverse) != 0) { rc = -1; break; }
         std::cout << "Saved checkpoint at step " << last + 1 << std::endl;
       }
       ++step;
@@ -819,6 +846,28 @@ struct MultiphaseLBM {
     std::cout << std::endl;
     mplbm_destroy(ctx);
     return rc;
+  }
+
+  // Driver state saved beside a checkpoint (<checkpoint>.driver, one line
+  // "reverse R"): the injection direction is not part of the populations.
+  // A checkpoint without the file resumes injecting.
+  static int write_driver_state(const std::string& ckpt, int reverse) {
+    const std::string name = ckpt + ".driver";
+    FILE* f = std::fopen(name.c_str(), "w");
+    if (!f) return set_err("cannot write %s", name.c_str());
+    const bool ok = std::fprintf(f, "reverse %d\n", reverse) > 0;
+    if (std::fclose(f) != 0 || !ok) return set_err("write error on %s", name.c_str());
+    return 0;
+  }
+  static int read_driver_state(const std::string& ckpt, int* reverse) {
+    const std::string name = ckpt + ".driver";
+    std::ifstream f(name);
+    if (!f) return 0;
+    std::string key;
+    int r = 0;
+    if (!(f >> key >> r) || key != "reverse" || (r != 1 && r != -1)) return set_err("%s is malformed", name.c_str());
+    *reverse = r;
+    return 0;
   }
 
   void write_stats_row(int64_t step, const mplbm_cluster_stats& s) {  // :1057-1083
@@ -909,6 +958,7 @@ bool MultiphaseLBM::mplbm_parse_scalar_token(std::istream& file, const std::stri
     file >> s;
     in.rounding = s == "stochastic" ? MPLBM_ROUND_STOCHASTIC : MPLBM_ROUND_NEAREST;
   }
+  else if (token == "saturation_cycle") file >> in.saturation_cycle;
   else if (token == "read_checkpoint") {
     file >> s;
     std::snprintf(in.checkpoint_filename, sizeof in.checkpoint_filename, "%s", s.c_str());
@@ -1155,6 +1205,24 @@ int mplbm_host_write_vtk(const char* path, const uint8_t* is_fluid, const float*
                          const int64_t* liq, const int64_t* gas, const int64_t* solid, int64_t lx, int64_t ly,
                          int64_t lz) {
   return write_vtk(path, Dims{lx, ly, lz}, is_fluid, rho, pres, liq, gas, solid);
+}
+
+int mplbm_host_read_data(const char* input_path, mplbm_input* in, uint8_t* is_fluid, float* rho, int64_t capacity,
+                         int64_t* eroded) {
+  if (!input_path || !in) return set_err("null argument");
+  MultiphaseLBM box;
+  if (box.read_data(input_path) != 0) return -1;
+  *in = box.in;
+  in->lx = box.d.lx;
+  in->ly = box.d.ly;
+  in->lz = box.d.lz;
+  const int64_t n = (int64_t)box.d.n();
+  if ((is_fluid || rho) && n > capacity) return set_err("domain of %lld cells exceeds capacity %lld", (long long)n,
+                                                         (long long)capacity);
+  if (is_fluid) std::copy(box.is_fluid.begin(), box.is_fluid.end(), is_fluid);
+  if (rho) std::copy(box.rho.begin(), box.rho.end(), rho);
+  if (eroded) *eroded = box.eroded_nodes;
+  return 0;
 }
 
 int mplbm_host_run(const char* input_path) {

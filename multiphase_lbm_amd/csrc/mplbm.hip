@@ -1510,6 +1510,43 @@ __global__ void k_write_pops(Lattice<Mo> L, Phys<typename Mo::C> ph, const doubl
   for (int i = 0; i < kQ; ++i) compact_store_all(L, L.fout, i, x, r, encode<Mo>((C)in[(uint64_t)c * kQ + i], i, ph));
 }
 
+// Macro fields and psi of every cell from the populations of fin, for a
+// context whose populations were loaded from a checkpoint: solid cells as
+// k_init leaves them (rho = rhos, p = 0, psi = Psi(rhos), u = 0); a fluid
+// cell gets rho = sum f (ascending i), P(rho), Psi(rho) and u = sum f e / rho.
+template <class Mo>
+__global__ void k_macro_from_pops(Lattice<Mo> L, Phys<typename Mo::C> ph) {
+  using C = typename Mo::C;
+  const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+  if (c >= L.ncell) return;
+  if (L.link[c] & 1u) {
+    L.rho[c] = (typename Mo::M)ph.rhos;
+    L.pres[c] = (typename Mo::M)0;
+    L.psi[c] = psi_of<Mo>(ph, ph.rhos);
+    L.u[c] = 0;
+    L.u[(uint64_t)L.ncell + c] = 0;
+    L.u[2ull * L.ncell + c] = 0;
+    return;
+  }
+  const float sh = own_shift(L, ph, c);
+  C rho = (C)0, m[3] = {(C)0, (C)0, (C)0};
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) {
+    const C f = decode_own<Mo>(L.fin[any_pix(L, i, c)], i, ph, sh);
+    rho = rho + f;
+    m[0] = m[0] + f * (C)ex(i);
+    m[1] = m[1] + f * (C)ey(i);
+    m[2] = m[2] + f * (C)ez(i);
+  }
+  L.rho[c] = (typename Mo::M)rho;
+  L.pres[c] = (typename Mo::M)pressure_of<Mo>(ph, rho);
+  L.psi[c] = psi_of<Mo>(ph, rho);
+  const C inv = (C)(1.0 / (double)rho);
+  L.u[c] = (typename Mo::M)(m[0] * inv);
+  L.u[(uint64_t)L.ncell + c] = (typename Mo::M)(m[1] * inv);
+  L.u[2ull * L.ncell + c] = (typename Mo::M)(m[2] * inv);
+}
+
 // Stored values (no decoding) of the 19 populations of every cell of plane
 // x, [y][z][19], solid cells 0: gather from fin, or scatter to fout (own
 // slots and the compact layout's ghost planes).
@@ -2053,7 +2090,12 @@ static mplbm_status launch_step(mplbm_ctx* ctx, int64_t step, int32_t reverse, u
       launch_as(std::false_type{}, std::false_type{}, R, faces);
     };
     const bool x_faces_only = ph.any_face && !(ph.face[2] | ph.face[3] | ph.face[4] | ph.face[5]);
-    if (x_faces_only && T.nseg >= 3 && T.seg >= 2) {
+    // a middle segment [x0, x1) pulls planes x0-1 .. x1 for its psi ring, so
+    // the plain variant may run it only if neither ring plane is a face
+    // plane: x0-1 >= 1 needs seg >= 2, x1 <= lx-2 needs a last segment of at
+    // least two planes
+    const int last_len = (int)ctx->cfg.lx - (T.nseg - 1) * T.seg;
+    if (x_faces_only && T.nseg >= 3 && T.seg >= 2 && last_len >= 2) {
       // face rules and the skipped force only touch planes 0 and lx-1: the
       // first and last segments run the face variant, the others the plain
       // one (fewer registers), in two launches
@@ -2642,17 +2684,21 @@ mplbm_status mplbm_load_checkpoint(mplbm_ctx* ctx, const char* path, int64_t* st
     }, true);
     if (st != MPLBM_OK) return st;
   }
-  if (!ctx->compact) {  // bounce-back slots of the full layout
-    st = dispatch(ctx, [&](auto mo) -> mplbm_status {
-      using Mo = decltype(mo);
-      auto L = make_lattice<Mo>(ctx, ctx->cur, ctx->cur);
-      hipLaunchKernelGGL(k_ghost<Mo>, dim3((unsigned)((ctx->ncell + 255) / 256)), dim3(256), 0, ctx->stream, L);
-      return MPLBM_OK;
-    });
-    if (st != MPLBM_OK) return st;
-    HIP_TRY(ctx, hipGetLastError());
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  }
+  // bounce-back slots of the full layout, then every macro field and psi
+  // (the two-pass kernel reads psi of solid neighbours; a context that was
+  // never initialised holds no fields yet)
+  st = dispatch(ctx, [&](auto mo) -> mplbm_status {
+    using Mo = decltype(mo);
+    const auto ph = make_phys<typename Mo::C>(ctx, 0, 1, 0, false);
+    auto L = make_lattice<Mo>(ctx, ctx->cur, ctx->cur);
+    const dim3 grid((unsigned)((ctx->ncell + 255) / 256));
+    if (!ctx->compact) hipLaunchKernelGGL(k_ghost<Mo>, grid, dim3(256), 0, ctx->stream, L);
+    hipLaunchKernelGGL(k_macro_from_pops<Mo>, grid, dim3(256), 0, ctx->stream, L, ph);
+    return MPLBM_OK;
+  });
+  if (st != MPLBM_OK) return st;
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   ctx->have_fields = true;
   if (step) *step = hd.step;
   return MPLBM_OK;
@@ -2744,11 +2790,77 @@ namespace {
 struct Compat {
   mplbm_config cfg{};
   bool have_domain = false, have_params = false;
+  int abort_on_error = 1;
+  int status = MPLBM_OK;
   mplbm_ctx* ctx = nullptr;
-  float* rho = nullptr;
-  float* pres = nullptr;
+  void* rho = nullptr;   // the caller's density and pressure arrays (any memory kind)
+  void* pres = nullptr;
+  std::vector<float> hrho, hpres;
 };
 Compat g_compat;
+
+// The reference's cudaErrorCheck (include/iteration_host.h:6-15): print the
+// error, then exit with its code unless the caller asked to continue.
+// src: the context whose last error describes the failure, or null for
+// failures recorded without one.
+void compat_report(mplbm_status st, const mplbm_ctx* src = nullptr) {
+  g_compat.status = st;
+  if (st == MPLBM_OK) return;
+  fprintf(stderr, "mplbm compat error %d: %s\n", (int)st, mplbm_last_error(src));
+  if (g_compat.abort_on_error) exit((int)st);
+}
+
+// Copies density and pressure from the context into the caller's arrays.
+mplbm_status compat_write_back(mplbm_ctx* ctx) {
+  const uint64_t n = ctx->ncell;
+  g_compat.hrho.resize(n);
+  g_compat.hpres.resize(n);
+  mplbm_status st = mplbm_read_fields(ctx, g_compat.hrho.data(), g_compat.hpres.data(), nullptr);
+  if (st != MPLBM_OK) return st;
+  HIP_TRY(ctx, hipMemcpy(g_compat.rho, g_compat.hrho.data(), n * sizeof(float), hipMemcpyDefault));
+  HIP_TRY(ctx, hipMemcpy(g_compat.pres, g_compat.hpres.data(), n * sizeof(float), hipMemcpyDefault));
+  return MPLBM_OK;
+}
+
+mplbm_status compat_initialization(void* rho_dd, void* pressure_dd, void* is_fluid_dd) {
+  if (!rho_dd || !pressure_dd || !is_fluid_dd)
+    return fail(nullptr, MPLBM_ERR_ARG, "multiphaselbm_initialization: null rho, pressure or is_fluid address");
+  void* rho = *static_cast<void**>(rho_dd);
+  void* pres = *static_cast<void**>(pressure_dd);
+  const void* fluid = *static_cast<void**>(is_fluid_dd);
+  if (!rho || !pres || !fluid)
+    return fail(nullptr, MPLBM_ERR_ARG, "multiphaselbm_initialization: null rho, pressure or is_fluid array");
+  if (!g_compat.have_params) return fail(nullptr, MPLBM_ERR_STATE, "transfer_parameter must come first");
+  mplbm_compat_release();
+  mplbm_ctx* ctx = nullptr;
+  mplbm_status st = mplbm_create(&g_compat.cfg, &ctx);
+  if (st != MPLBM_OK) return st;
+  const uint64_t n = ctx->ncell;
+  auto bail = [&](mplbm_status s) {
+    g_create_error = ctx->err;
+    mplbm_destroy(ctx);
+    return s;
+  };
+  static_assert(sizeof(bool) == 1, "the reference's Boolxyz mask is one byte per cell");
+  std::vector<uint8_t> m(n);
+  std::vector<float> r(n);
+  if (hipMemcpy(m.data(), fluid, n, hipMemcpyDefault) != hipSuccess ||
+      hipMemcpy(r.data(), rho, n * sizeof(float), hipMemcpyDefault) != hipSuccess) {
+    fail(ctx, MPLBM_ERR_HIP, "multiphaselbm_initialization: cannot read the caller's rho or is_fluid");
+    return bail(MPLBM_ERR_HIP);
+  }
+  for (uint64_t k = 0; k < n; ++k) m[k] = m[k] ? 1 : 0;
+  if ((st = mplbm_set_geometry(ctx, m.data())) != MPLBM_OK || (st = mplbm_init_fields(ctx, r.data())) != MPLBM_OK)
+    return bail(st);
+  g_compat.ctx = ctx;
+  g_compat.rho = rho;
+  g_compat.pres = pres;
+  if ((st = compat_write_back(ctx)) != MPLBM_OK) {
+    g_compat.ctx = nullptr;
+    return bail(st);
+  }
+  return MPLBM_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -2763,13 +2875,15 @@ int mplbm_compat_set_domain(int64_t lx, int64_t ly, int64_t lz, int32_t storage)
   return MPLBM_OK;
 }
 
-int transfer_parameter(const float* tau, const float* G, const bool* carn_star, const float* T_Tc,
-                       const float* rhol_spinodal, const float* rhog_spinodal, const bool* inject_density,
-                       const size_t* rho_inject_period, const float* rho_increment, const float* rhos,
-                       const float* inject_position, const size_t* statPeriod) {
+void transfer_parameter(const float* tau, const float* G, const bool* carn_star, const float* T_Tc,
+                        const float* rhol_spinodal, const float* rhog_spinodal, const bool* inject_density,
+                        const size_t* rho_inject_period, const float* rho_increment, const float* rhos,
+                        const float* inject_position, const size_t* statPeriod) {
   if (!tau || !G || !carn_star || !T_Tc || !rhol_spinodal || !rhog_spinodal || !inject_density ||
-      !rho_inject_period || !rho_increment || !rhos || !inject_position || !statPeriod)
-    return fail(nullptr, MPLBM_ERR_ARG, "transfer_parameter: null argument");
+      !rho_inject_period || !rho_increment || !rhos || !inject_position || !statPeriod) {
+    compat_report(fail(nullptr, MPLBM_ERR_ARG, "transfer_parameter: null argument"));
+    return;
+  }
   if (!g_compat.have_domain) {
     mplbm_config_default(&g_compat.cfg);
     g_compat.have_domain = true;
@@ -2789,40 +2903,34 @@ int transfer_parameter(const float* tau, const float* G, const bool* carn_star, 
   c.inject_position = *inject_position;
   (void)statPeriod;  // copied to the device but unused there (iteration_host.cu:39)
   g_compat.have_params = true;
-  return MPLBM_OK;
+  compat_report(MPLBM_OK);
 }
 
-int multiphaselbm_initialization(float* rho, float* pressure, const bool* is_fluid) {
-  if (!rho || !pressure || !is_fluid) return fail(nullptr, MPLBM_ERR_ARG, "multiphaselbm_initialization: null");
-  if (!g_compat.have_params) return fail(nullptr, MPLBM_ERR_STATE, "transfer_parameter must come first");
-  mplbm_compat_release();
-  mplbm_ctx* ctx = nullptr;
-  mplbm_status st = mplbm_create(&g_compat.cfg, &ctx);
-  if (st != MPLBM_OK) return st;
-  const uint64_t n = ctx->ncell;
-  std::vector<uint8_t> m(n);
-  for (uint64_t k = 0; k < n; ++k) m[k] = is_fluid[k] ? 1 : 0;
-  if ((st = mplbm_set_geometry(ctx, m.data())) != MPLBM_OK || (st = mplbm_init_fields(ctx, rho)) != MPLBM_OK ||
-      (st = mplbm_read_fields(ctx, rho, pressure, nullptr)) != MPLBM_OK) {
-    g_create_error = ctx->err;
-    mplbm_destroy(ctx);
-    return st;
-  }
-  g_compat.ctx = ctx;
-  g_compat.rho = rho;
-  g_compat.pres = pressure;
-  return MPLBM_OK;
+void multiphaselbm_initialization(void* rho_dd, void* psi_dd, void* pressure_dd, void* is_fluid_dd,
+                                  void* collision_f_dd, void* stream_f_dd, void* force_dd) {
+  // psi, both lattices and the force are owned by the library
+  (void)psi_dd;
+  (void)collision_f_dd;
+  (void)stream_f_dd;
+  (void)force_dd;
+  compat_report(compat_initialization(rho_dd, pressure_dd, is_fluid_dd));
 }
 
-int multiphaselbm_iteration(long step, size_t statPeriod, int reverse, unsigned int inject_type) {
+void multiphaselbm_iteration(long step, size_t statPeriod, int reverse, unsigned int inject_type) {
   mplbm_ctx* ctx = g_compat.ctx;
-  if (!ctx) return fail(nullptr, MPLBM_ERR_STATE, "multiphaselbm_initialization must come first");
+  if (!ctx) {
+    compat_report(fail(nullptr, MPLBM_ERR_STATE, "multiphaselbm_initialization must come first"));
+    return;
+  }
   const bool stat = statPeriod > 0 && (uint64_t)step % statPeriod == 0;
   mplbm_status st = mplbm_step(ctx, step, reverse, inject_type, 1, stat ? MPLBM_STEP_WRITE_MACRO : 0);
-  if (st != MPLBM_OK) return st;
-  if (stat) return mplbm_read_fields(ctx, g_compat.rho, g_compat.pres, nullptr);
-  return MPLBM_OK;
+  if (st == MPLBM_OK && stat) st = compat_write_back(ctx);
+  compat_report(st, ctx);
 }
+
+void mplbm_compat_set_abort(int abort_on_error) { g_compat.abort_on_error = abort_on_error ? 1 : 0; }
+
+int mplbm_compat_status(void) { return g_compat.status; }
 
 void mplbm_compat_release(void) {
   if (g_compat.ctx) mplbm_destroy(g_compat.ctx);

@@ -41,6 +41,7 @@ class Input(ctypes.Structure):
         ("host_statistics", ctypes.c_int32), ("read_checkpoint", ctypes.c_int32),
         ("checkpointPeriod", I64), ("checkpoint_filename", ctypes.c_char * 256),
         ("rounding", ctypes.c_int32), ("device_voxeliser", ctypes.c_int32), ("shift", ctypes.c_int32),
+        ("saturation_cycle", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -85,6 +86,8 @@ SIGNATURES = {
                                                 I64, VP, VP, ctypes.POINTER(ClusterStats)]),
     "mplbm_host_write_vtk": (ctypes.c_int, [ctypes.c_char_p, VP, VP, VP, VP, VP, VP, I64, I64, I64]),
     "mplbm_host_write_vtr": (ctypes.c_int, [ctypes.c_char_p, VP, VP, VP, VP, VP, VP, I64, I64, I64]),
+    "mplbm_host_read_data": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(Input), VP, VP, I64,
+                                            ctypes.POINTER(I64)]),
     "mplbm_host_run": (ctypes.c_int, [ctypes.c_char_p]),
 }
 
@@ -118,6 +121,24 @@ def parse_input(path) -> dict:
     if load().mplbm_host_parse(str(path).encode(), ctypes.byref(inp)) != 0:
         raise _err("parse")
     return inp.as_dict()
+
+
+def read_data(path):
+    """The reference's read_data (tokens, domain, density initialisation,
+    geometry, erosion): returns (input dict, is_fluid uint8 [x][y][z],
+    initial rho float32 [x][y][z], number of eroded solid nodes)."""
+    lib = load()
+    inp = Input()
+    if lib.mplbm_host_read_data(str(path).encode(), ctypes.byref(inp), None, None, 0, None) != 0:
+        raise _err("read_data")
+    shape = (inp.lx, inp.ly, inp.lz)
+    fluid = np.empty(shape, np.uint8)
+    rho = np.empty(shape, np.float32)
+    eroded = I64()
+    if lib.mplbm_host_read_data(str(path).encode(), ctypes.byref(inp), _p(fluid), _p(rho), fluid.size,
+                                ctypes.byref(eroded)) != 0:
+        raise _err("read_data")
+    return inp.as_dict(), fluid, rho, eroded.value
 
 
 def place_sphere(is_fluid, x, y, z, R, solid_number=None, solid_id=0):

@@ -39,15 +39,27 @@ def test_restart_continues_bit_identically(tmp_path, storage, kernel, bb):
         a.init_fields(rho)
         a.step(0, 5)
         a.save_checkpoint(path, 5)
+        f5 = a.read_populations()
+        # a context whose freed device memory holds NaN fields: a field that
+        # the load leaves unwritten in the next context shows up as NaN
+        with mp.Context(*shape, **kw) as junk:
+            junk.set_geometry(fluid)
+            junk.init_fields(np.full_like(rho, np.nan))
+        # context a stays alive, so b cannot inherit its fields
+        with mp.Context(*shape, **kw) as b:
+            b.set_geometry(fluid)
+            assert b.load_checkpoint(path) == 5
+            r0, p0, u0 = b.read_fields()
+            assert np.isfinite(r0).all() and np.isfinite(p0).all() and np.isfinite(u0).all()
+            rhos = SC["rhos"] if storage == "fp64" else float(np.float32(SC["rhos"]))
+            assert np.all(r0[~fluid] == rhos)
+            assert np.allclose(r0[fluid], f5[fluid].sum(-1), rtol=1e-6)
+            b.step(5, 6, write_macro=True)
+            fb = b.read_populations()
+            rb, _, _ = b.read_fields()
         a.step(5, 6, write_macro=True)
         fa = a.read_populations()
         ra, _, _ = a.read_fields()
-    with mp.Context(*shape, **kw) as b:
-        b.set_geometry(fluid)
-        assert b.load_checkpoint(path) == 5
-        b.step(5, 6, write_macro=True)
-        fb = b.read_populations()
-        rb, _, _ = b.read_fields()
     assert np.array_equal(fa[fluid], fb[fluid])
     assert np.array_equal(ra[fluid], rb[fluid])
 

@@ -123,25 +123,56 @@ inject_type 0
 inject_position 1
 max_saturation 0.5
 min_saturation 0.3
-step_max 3001
+step_max {step_max}
 kernel compact
+{extra}
 """
 
 
-def test_hysteresis_cycles_injection_and_drainage(tmp_path):
-    """max_saturation / min_saturation (parsed, never acted on in the
-    reference): the driver injects until the saturation reaches the maximum,
-    drains to the minimum, then injects again."""
-    folder = tmp_path / "hyst"
+def _run_hyst(tmp_path, tag, step_max=3001, extra="saturation_cycle 1"):
+    folder = tmp_path / f"hyst{tag}"
     folder.mkdir()
-    inp = tmp_path / "hyst.txt"
-    inp.write_text(HYSTERESIS.format(folder=folder))
-    print("case driver hysteresis", flush=True)
+    inp = tmp_path / f"hyst{tag}.txt"
+    inp.write_text(HYSTERESIS.format(folder=folder, step_max=step_max, extra=extra))
+    print(f"case driver hysteresis {tag} {extra!r}", flush=True)
     assert harness.load().mplbm_host_run(str(inp).encode()) == 0, harness.load().mplbm_host_last_error()
-    sat = np.array([float(r[8]) for r in _rows(folder)])
+    return folder
+
+
+def _sat(folder):
+    return np.array([float(r[8]) for r in _rows(folder)])
+
+
+def test_hysteresis_cycles_injection_and_drainage(tmp_path):
+    """max_saturation / min_saturation with saturation_cycle 1 (the
+    reference parses both and never acts on them): the driver injects until
+    the saturation reaches the maximum, drains to the minimum, then injects
+    again.  A checkpoint taken while draining resumes draining, so the
+    resumed run writes the uninterrupted run's Stats.txt rows.  Without
+    saturation_cycle the reference's input keeps injecting."""
+    full = _run_hyst(tmp_path, "_full")
+    sat = _sat(full)
+    steps = [int(r[0]) for r in _rows(full)]
     assert np.isfinite(sat).all()
     i = int(np.argmax(sat >= 0.5))
     assert sat[i] >= 0.5, "never reached max_saturation"
     j = i + int(np.argmax(sat[i:] <= 0.3))
     assert sat[j] <= 0.3, "never drained to min_saturation"
     assert sat[j:].max() > sat[j] + 0.05, "injection did not resume"
+    # checkpoint two statistics steps into the drainage, then resume
+    assert j >= i + 3
+    ck = steps[i + 2]
+    first = _run_hyst(tmp_path, "_a", step_max=ck + 1, extra=f"saturation_cycle 1\ncheckpointPeriod {ck}")
+    assert (first / "checkpoint.mplbm").exists()
+    assert (first / "checkpoint.mplbm.driver").read_text().split() == ["reverse", "-1"]
+    resumed = _run_hyst(tmp_path, "_b", extra=f"saturation_cycle 1\nread_checkpoint {first / 'checkpoint.mplbm'}")
+    rows_full = {r[0]: r for r in _rows(full)}
+    rows_res = _rows(resumed)
+    assert rows_res[0][0] == str(ck) and len(rows_res) == len(steps) - (i + 2)
+    for r in rows_res:
+        assert r == rows_full[r[0]], (r, rows_full[r[0]])
+    # the reference's behaviour: the same input without saturation_cycle
+    plain = _run_hyst(tmp_path, "_plain", step_max=steps[i + 4] + 1, extra="")
+    sp = _sat(plain)
+    assert np.array_equal(sp[:i + 1], sat[:i + 1])
+    assert sp[-1] > sp[i] and sp[-1] > sat[i + 4], "drained without saturation_cycle"

@@ -233,6 +233,22 @@ def test_pressure_faces_and_body_force(kernel, seg):
     assert np.array_equal(g_u[fluid], st.u[fluid].astype(np.float64))
 
 
+@pytest.mark.parametrize("lx,seg", [(15, 7), (29, 7), (16, 7), (17, 8)])
+def test_pressure_faces_short_last_segment(lx, seg):
+    """x faces with a last segment of one plane (lx = k*seg + 1: the plane
+    before it is the outlet face, so no segment may run the plain variant
+    over it) and of two planes (the split into plain and face launches
+    stays on)."""
+    shape = (lx, 12, 10)
+    fluid, rho = _case(shape, 2, 2.0, 3.0, seed=lx, rho0=1.0, amp=0.05)
+    print(f"case pressure faces lx {lx} seg {seg}", flush=True)
+    params = dict(SC, G=-3.0, face_type=(1, 1, 0, 0, 0, 0), face_rho=(1.05, 0.95, 0, 0, 0, 0), seg_x=seg)
+    st, (g_rho, g_p, g_u, g_f, _) = _run_pair(shape, fluid, rho, 12, "fp32", "compact", **params)
+    assert np.isfinite(st.fA[fluid]).all()
+    assert np.array_equal(g_f[fluid], st.fA[fluid].astype(np.float64))
+    assert np.array_equal(g_rho[fluid], st.rho[fluid].astype(np.float64))
+
+
 @pytest.mark.parametrize("kernel", ["fused", "compact"])
 def test_checkpoint_roundtrip_continues_identically(kernel):
     shape = (12, 12, 12)

@@ -37,6 +37,7 @@ def test_parse_input_inject_txt():
     assert d["read_densities"] == 1 and d["densities_filename"] == "./hyteresis_res2_/result000002.vtk"
     assert d["rho0"] == 0.15 and d["rho_inject_period"] == 200 and d["rho_increment"] == 0.0015
     assert d["hysteresis"] == 1 and d["max_saturation"] == 0.95 and d["min_saturation"] == 0.3
+    assert d["saturation_cycle"] == 0  # the reference's input never changes the injection direction
     assert d["carn_star"] == 1
 
 
@@ -287,3 +288,114 @@ def test_read_sample_refusals(tmp_path):
         H.read_sample(q, (4, 4, 4))
     with pytest.raises(RuntimeError, match="Cannot open"):
         H.read_sample(tmp_path / "missing.txt", (4, 4, 4))
+
+
+# --- erosion (erode_solid_touch_liquid, src/multiphase_lbm.cu:842-890) -----
+
+EROSION_INPUT = """result_folder {folder}
+lx 24
+ly 20
+lz 16
+rho0 0.7
+IniPerturbRate 0.0285714
+resize
+read_positions {positions}
+{extra}
+"""
+
+# three spheres x, y, z, r (read_positions computes the packing density that
+# the erosion count starts from, :684-691; place_sphere does not)
+EROSION_SPHERES = [(8, 9, 7, 6.5), (18, 6, 10, 5), (3, 17, 2, 4)]
+
+EX, EY, EZ = (np.array(v) for v in (
+    [0, -1, 0, 0, -1, -1, -1, -1, 0, 0, 1, 0, 0, 1, 1, 1, 1, 0, 0],
+    [0, 0, -1, 0, -1, 1, 0, 0, -1, -1, 0, 1, 0, 1, -1, 0, 0, 1, 1],
+    [0, 0, 0, -1, 0, 0, -1, 1, -1, 1, 0, 0, 1, 0, 0, 1, -1, 1, -1]))
+
+
+def _touching(fluid):
+    """find_solid_touch_liquid (:810-838): a solid node with a fluid node at
+    x - e_i for some i = 1..18, periodic."""
+    up_fluid = np.zeros_like(fluid)
+    for i in range(1, 19):
+        up_fluid |= np.roll(fluid, shift=(EX[i], EY[i], EZ[i]), axis=(0, 1, 2))
+    return up_fluid & ~fluid
+
+
+def _erode_replay(fluid, target):
+    """Brute-force erosion: the reference's float count, whole layers while
+    the count exceeds the touching solids (recounted after each layer), then
+    one random pass in z, y, x order on glibc rand() after the density
+    initialisation's srand(1) and one draw per cell (:134-144).  Returns the
+    mask, the number eroded and the number of whole layers."""
+    fluid = fluid.copy()
+    solids = int((~fluid).sum())
+    pd = np.float32(solids) / np.float32(fluid.size)
+    count = int((np.float32(1) - np.float32(target) / pd) * np.float32(solids))
+    touch = _touching(fluid)
+    done, layers = 0, 0
+    while count > touch.sum():
+        n = int(touch.sum())
+        fluid |= touch
+        count -= n
+        done += n
+        layers += 1
+        touch = _touching(fluid)
+        if not touch.any():
+            break
+    if 0 < count <= touch.sum():
+        rate = np.float32(count / float(touch.sum()))
+        libc = ctypes.CDLL("libc.so.6")
+        libc.srand(1)
+        for _ in range(fluid.size):
+            libc.rand()
+        for z in range(fluid.shape[2]):
+            for y in range(fluid.shape[1]):
+                for x in range(fluid.shape[0]):
+                    if touch[x, y, z] and np.float32(libc.rand()) / np.float32(2147483647) < rate:
+                        fluid[x, y, z] = True
+                        done += 1
+    return fluid, done, layers
+
+
+def _read(tmp_path, tag, extra):
+    pos = tmp_path / "spheres.txt"
+    pos.write_text("".join(f"{v}\n" for sph in EROSION_SPHERES for v in sph))
+    p = tmp_path / f"erosion_{tag}.txt"
+    p.write_text(EROSION_INPUT.format(folder=tmp_path, positions=pos, extra=extra))
+    return H.read_data(p)
+
+
+@pytest.mark.parametrize("layers_wanted", [0, 1, 2])
+def test_erosion_against_brute_force(tmp_path, layers_wanted):
+    """0: the target lies within the first layer of touching solids (the
+    reference's own path: one random pass whose draws follow its rand()
+    stream); 1, 2: the target lies beyond one or two whole layers, where the
+    count is recounted after each layer (the reference's size_t counter is
+    not; see erode() in the harness)."""
+    _, fluid0, _, e0 = _read(tmp_path, "none", "")
+    assert e0 == 0
+    fluid0 = fluid0.astype(bool)
+    solids = int((~fluid0).sum())
+    pd = np.float32(solids) / np.float32(fluid0.size)
+    # layer sizes of successive whole-layer erosion
+    sizes, f = [], fluid0.copy()
+    for _ in range(3):
+        t = _touching(f)
+        sizes.append(int(t.sum()))
+        f |= t
+    # an erosion count halfway into the wanted layer
+    want = sum(sizes[:layers_wanted]) + sizes[layers_wanted] // 2
+    target = float(pd * np.float32(1 - want / solids))
+    d, fluid, rho, eroded = _read(tmp_path, f"t{layers_wanted}", f"target_density {target!r}")
+    assert d["erosion"] == 1
+    expect, done, layers = _erode_replay(fluid0, d["target_density"])
+    assert layers == layers_wanted
+    assert np.array_equal(fluid.astype(bool), expect)
+    assert eroded == done == int((fluid.astype(bool) & ~fluid0).sum())
+    # eroded nodes are near the requested count (random pass: binomial)
+    sd = math.sqrt(sizes[layers_wanted] * 0.25)
+    assert abs(done - want) <= 5 * sd + 2
+    assert np.all(fluid.astype(bool) >= fluid0)  # erosion only turns solids fluid
+    # the density initialisation is untouched by the erosion
+    assert np.array_equal(rho, H.rho_init(fluid.shape, 0.7, 0.0285714))

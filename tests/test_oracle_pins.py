@@ -298,3 +298,132 @@ def test_poiseuille_curvature():
     assert abs(-2 * c[0] * nu / g - 1.0) < 0.02
     resid = u - np.polyval(c, y)
     assert np.abs(resid).max() < 1e-3 * u.max()
+
+
+# --- pressure faces (anti-bounce-back, iteration_device.cuh:213-243, 282-286)
+
+def test_pressure_faces_at_rest_keep_the_face_density():
+    """Fluid at rest with density rho_w everywhere and both x faces at rho_w:
+    the entering populations -A_opp + 2 t_i rho_w (1 + 4.5 (e.u)^2 - 1.5 u^2)
+    are t_i rho_w again, so every population stays t_i rho_w (to rounding)
+    for 200 steps.  A wrong sign of A_opp gives 3 t_i rho_w, a dropped factor
+    2 gives 0, and both move the state at the first step."""
+    shape = (12, 5, 4)
+    rw = 1.3
+    P = oracle.Params(*shape, G=0.0, tau=0.9, face_type=(1, 1, 0, 0, 0, 0), face_rho=(rw, rw, 0, 0, 0, 0))
+    st = oracle.State(P, np.ones(shape, bool), np.full(shape, rw), dtype=np.float64)
+    w = oracle.tables()[5]
+    st.run(200)
+    assert np.allclose(st.fA, w * rw, rtol=1e-13, atol=0)
+    assert np.allclose(st.rho, rw, rtol=1e-13)
+
+
+def test_pressure_driven_channel():
+    """G = 0 channel between halfway bounce-back walls (y = 0 and ly - 1
+    solid), pressure faces rho_in = 1.01 at x = 0 and rho_out = 0.99 at
+    x = lx - 1, periodic z.  At steady state:
+    * the mass flux runs from the high to the low density face and is the
+      same through every cross-section (mass conservation);
+    * the cross-section density is linear in x and extrapolates to rho_in
+      and rho_out half a node outside the face nodes, where anti-bounce-back
+      imposes them;
+    * the velocity profile is Poiseuille's: nu rho u'' = c_s^2 drho/dx."""
+    lx, ly, lz = 48, 12, 2
+    shape = (lx, ly, lz)
+    fluid = np.ones(shape, bool)
+    fluid[:, 0, :] = fluid[:, -1, :] = False
+    rin, rout, tau = 1.01, 0.99, 1.0
+    P = oracle.Params(*shape, G=0.0, tau=tau, rhos=1.0, face_type=(1, 1, 0, 0, 0, 0),
+                      face_rho=(rin, rout, 0, 0, 0, 0))
+    st = oracle.State(P, fluid, np.ones(shape), dtype=np.float64)
+    st.run(6000)
+    e = np.stack([EX, EY, EZ], 1).astype(float)
+    f = st.fA[:, 1:-1, 0, :]                      # fluid rows [x][y][19]
+    rho = f.sum(-1)
+    jx = (f @ e)[..., 0]                          # x momentum (no force, G = 0)
+    flux = jx.sum(1)                              # through each cross-section
+    assert np.all(flux > 0)
+    assert np.ptp(flux) <= 1e-6 * flux.mean()
+    rbar = rho.mean(1)
+    x = np.arange(lx, dtype=float)
+    inner = slice(4, lx - 4)
+    c = np.polyfit(x[inner], rbar[inner], 1)
+    assert np.abs(rbar[inner] - np.polyval(c, x[inner])).max() < 1e-3 * (rin - rout)
+    assert abs(np.polyval(c, -0.5) - rin) < 0.02 * (rin - rout)
+    assert abs(np.polyval(c, lx - 0.5) - rout) < 0.02 * (rin - rout)
+    # Poiseuille curvature in the middle of the channel
+    xm = lx // 2
+    u = jx[xm] / rho[xm]
+    y = np.arange(1, ly - 1, dtype=float)
+    q = np.polyfit(y, u, 2)
+    nu = (tau - 0.5) / 3
+    want = (c[0] / 3.0) / (nu * rho[xm].mean())   # u'' = c_s^2 (drho/dx) / (nu rho)
+    assert abs(2 * q[0] / want - 1) < 0.03
+
+
+# --- injection types 1 and 2 (iteration_device.cuh:47-59) -------------------
+
+@pytest.mark.parametrize("inject_type", [0, 1, 2])
+@pytest.mark.parametrize("reverse", [1, -1])
+def test_injection_mass_gain_against_brute_force(inject_type, reverse):
+    """One step from rest with a random density field: the post-collision
+    mass of a node is its streamed density plus inc/tau when it is eligible
+    (y < inject_position * ly and, for type 1, streamed density below
+    rhol_spinodal, for type 2 above), and unchanged otherwise.  The streamed
+    density is formed here by brute force (a sum of rolled t_i rho)."""
+    shape = (7, 10, 6)
+    rng = np.random.default_rng(11 + inject_type)
+    rho0 = rng.uniform(0.5, 1.5, shape)
+    inc, tau, pos, thr = 0.02, 0.8, 0.45, 1.0
+    P = oracle.Params(*shape, G=0.0, tau=tau, rho_inject_period=3, rho_increment=inc, inject_position=pos,
+                      rhol_spinodal=thr)
+    w = oracle.tables()[5]
+    streamed = sum(w[i] * np.roll(rho0, shift=(EX[i], EY[i], EZ[i]), axis=(0, 1, 2)) for i in range(19))
+    y = np.arange(shape[1])[None, :, None]
+    elig = np.broadcast_to(y < pos * shape[1], shape)
+    if inject_type == 1:
+        elig = elig & (streamed < thr)
+    elif inject_type == 2:
+        elig = elig & (streamed > thr)
+    assert 0 < elig.sum() < elig.size
+    for step, fires in ((6, True), (7, False)):
+        st = oracle.State(P, np.ones(shape, bool), rho0, dtype=np.float64)
+        st.step = step
+        st.run(1, reverse=reverse, inject_type=inject_type)
+        gain = st.fA.sum(-1) - streamed
+        want = np.where(elig, reverse * inc / tau, 0.0) if fires else np.zeros(shape)
+        assert np.allclose(gain, want, rtol=0, atol=1e-14), (step, inject_type)
+        assert np.allclose(st.rho, streamed + (np.where(elig, reverse * inc, 0.0) if fires else 0.0), atol=1e-14)
+
+
+# --- solid psi (iteration_kernel.cuh:21-23, iteration_device.cuh:89-147) ----
+
+def test_solid_neighbours_carry_psi_of_rhos():
+    """Fluid at rest with density rho_f next to solids: with rhos = rho_f the
+    psi field is uniform and F = 0 on every fluid node; with rhos != rho_f the
+    force on each fluid node is -G psi(x) sum_i w_i psi(x + e_i) e_i with
+    psi = 1 - exp(-rhos) on solid neighbours, formed here by brute force."""
+    shape = (9, 8, 7)
+    fluid = np.ones(shape, bool)
+    fluid[3:6, 2:5, 2:4] = False
+    fluid[0, 7, 6] = False                       # a solid across the periodic corner
+    rf, G = 0.9, -5.0
+    w = oracle.tables()[5]
+    st = oracle.State(oracle.Params(*shape, G=G, rhos=rf), fluid, np.full(shape, rf), dtype=np.float64)
+    st.run(1)
+    assert np.abs(st.F[fluid]).max() < 1e-15
+    for rhos in (0.2, 1.7):
+        st = oracle.State(oracle.Params(*shape, G=G, rhos=rhos), fluid, np.full(shape, rf), dtype=np.float64)
+        st.run(1)
+        psi = np.where(fluid, 1 - np.exp(-st.rho), 1 - math.exp(-rhos))
+        F = np.zeros(shape + (3,))
+        for i in range(1, 19):
+            nb = np.roll(psi, shift=(-EX[i], -EY[i], -EZ[i]), axis=(0, 1, 2))   # psi(x + e_i)
+            F += (w[i] * nb)[..., None] * np.array([EX[i], EY[i], EZ[i]], float)
+        F *= (-G * psi)[..., None]
+        assert np.abs(F[fluid]).max() > 1e-2
+        assert np.allclose(st.F[fluid], F[fluid], rtol=0, atol=1e-13)
+        # solid psi enters with the sign of rhos - rho_f: the force on a
+        # fluid node next to a solid points towards it when rhos > rho_f
+        x, y, z = 2, 3, 2                         # fluid, solid at x + 1
+        assert np.sign(st.F[x, y, z, 0]) == np.sign(rhos - rf)
