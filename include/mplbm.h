@@ -293,6 +293,22 @@ mplbm_status mplbm_calc_cluster(mplbm_ctx* ctx, double rhol_spinodal, double rho
 mplbm_status mplbm_place_spheres(int64_t lx, int64_t ly, int64_t lz, const float* spheres, int64_t count,
                                  int64_t first_id, uint8_t* is_fluid, int64_t* solid_number);
 
+/* mplbm_place_spheres without any allocation, for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * is_fluid_dev (uint8 [x][y][z]) and solid_number_dev (int64 [x][y][z], may
+ * be NULL) are device memory of the caller, updated in place exactly as
+ * mplbm_place_spheres updates its host arrays; spheres is host memory as
+ * there.  The per-cell sphere numbers and the staged sphere list live in one
+ * static device buffer of 68 MiB per GPU (held until the process ends):
+ * at most 2^18 spheres per call, and at most 2^24 cells when
+ * solid_number_dev is given; larger calls are refused with MPLBM_ERR_ARG
+ * before any device work.  Runs on the null stream of the current device
+ * and synchronises it.  Calls that overlap on one GPU (other host threads)
+ * share the buffer and race.  Errors go to mplbm_last_error(NULL). */
+mplbm_status mplbm_place_spheres_noalloc(int64_t lx, int64_t ly, int64_t lz, const float* spheres, int64_t count,
+                                         int64_t first_id, uint8_t* is_fluid_dev, int64_t* solid_number_dev);
+
 /* Changes the FUSED or COMPACT kernel's tile (see mplbm_config.tile_y) and
  * planes per block; 0 keeps the tuned default for that field.  Used by the
  * autotuner (multiphase_lbm_amd/autotune.py); results do not depend on the

@@ -104,6 +104,8 @@ SIGNATURES = {
     "mplbm_calc_cluster": (ctypes.c_int, [VP, ctypes.c_double, ctypes.c_double, ctypes.POINTER(Clusters), VP, VP]),
     "mplbm_place_spheres": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, VP, ctypes.c_int64,
                                            ctypes.c_int64, VP, VP]),
+    "mplbm_place_spheres_noalloc": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, VP,
+                                                   ctypes.c_int64, ctypes.c_int64, VP, VP]),
     "mplbm_set_tiling": (ctypes.c_int, [VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "mplbm_info_get": (ctypes.c_int, [VP, ctypes.POINTER(Info)]),
     "mplbm_version": (ctypes.c_char_p, []),

@@ -40,6 +40,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -4144,6 +4145,23 @@ __global__ void k_vox_numbers(const uint32_t* __restrict__ ids, int64_t* __restr
 
 }  // namespace mplbm
 
+// Every axis of every sphere's box must hold a float-stepped sample run that
+// ends and fits k_voxelise's LDS: coordinates well inside float's integer
+// range and fewer than kVoxAxis samples.
+static mplbm_status check_spheres(const float* spheres, int64_t count) {
+  for (int64_t s = 0; s < count; ++s) {
+    const float* q = spheres + 4 * s;
+    for (int a = 0; a < 3; ++a) {
+      const float lo = q[a] - q[3], hi = q[a] + q[3];
+      if (!std::isfinite(q[a]) || !std::isfinite(q[3]) || std::fabs(lo) > 8388608.f || std::fabs(hi) > 8388608.f ||
+          (double)hi - (double)lo > (double)(kVoxAxis - 3))
+        return fail(nullptr, MPLBM_ERR_ARG, "place_spheres: sphere %ld (%g, %g, %g, R %g) outside the voxeliser's "
+                    "range (|coordinate| + R <= 2^23, R <= %d)", (long)s, q[0], q[1], q[2], q[3], (kVoxAxis - 3) / 2);
+    }
+  }
+  return MPLBM_OK;
+}
+
 extern "C" {
 
 mplbm_status mplbm_place_spheres(int64_t lx, int64_t ly, int64_t lz, const float* spheres, int64_t count,
@@ -4156,18 +4174,7 @@ mplbm_status mplbm_place_spheres(int64_t lx, int64_t ly, int64_t lz, const float
   if (count >= (int64_t)UINT32_MAX)
     return fail(nullptr, MPLBM_ERR_ARG, "place_spheres: %ld spheres in one call (max %u)", (long)count,
                 UINT32_MAX - 1);
-  for (int64_t s = 0; s < count; ++s) {
-    const float* q = spheres + 4 * s;
-    for (int a = 0; a < 3; ++a) {
-      // the float-stepped axis must end: coordinates well inside float's
-      // integer range and fewer than kVoxAxis samples
-      const float lo = q[a] - q[3], hi = q[a] + q[3];
-      if (!std::isfinite(q[a]) || !std::isfinite(q[3]) || std::fabs(lo) > 8388608.f || std::fabs(hi) > 8388608.f ||
-          (double)hi - (double)lo > (double)(kVoxAxis - 3))
-        return fail(nullptr, MPLBM_ERR_ARG, "place_spheres: sphere %ld (%g, %g, %g, R %g) outside the voxeliser's "
-                    "range (|coordinate| + R <= 2^23, R <= %d)", (long)s, q[0], q[1], q[2], q[3], (kVoxAxis - 3) / 2);
-    }
-  }
+  if (const mplbm_status cs = check_spheres(spheres, count); cs != MPLBM_OK) return cs;
   const uint64_t n = (uint64_t)lx * (uint64_t)ly * (uint64_t)lz;
   uint8_t* dmask = nullptr;
   uint32_t* dids = nullptr;
@@ -4207,6 +4214,96 @@ mplbm_status mplbm_place_spheres(int64_t lx, int64_t ly, int64_t lz, const float
   (void)hipFree(dsn);
   (void)hipFree(dsph);
   return st;
+}
+
+}  // extern "C"
+
+// ============================================================================
+// sphere voxeliser that allocates nothing (mplbm_place_spheres_noalloc)
+
+namespace mplbm {
+
+// Scratch of mplbm_place_spheres_noalloc: the per-cell sphere numbers
+// (uint32, cleared by every call that needs them) followed by the staged
+// sphere list (float4).  One copy per GPU, held until the process ends.
+constexpr uint64_t kNoAllocMaxCells = 1ull << 24;    // 256^3 cells of sphere numbers (64 MiB)
+constexpr uint64_t kNoAllocMaxSpheres = 1ull << 18;  // 262,144 spheres (4 MiB)
+constexpr uint64_t kNoAllocIdsBytes = kNoAllocMaxCells * sizeof(uint32_t);
+constexpr uint64_t kScratchBytes = kNoAllocIdsBytes + kNoAllocMaxSpheres * sizeof(float4);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+}  // namespace mplbm
+
+namespace {
+// Device address of `scratch` on each device, from hipGetSymbolAddress with
+// that device current; the symbol's host-side name is not a device address.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+mplbm_status scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(nullptr, MPLBM_ERR_HIP, "place_spheres_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return fail(nullptr, MPLBM_ERR_UNSUPPORTED, "device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(mplbm::scratch));
+    if (e != hipSuccess || !p)
+      return fail(nullptr, MPLBM_ERR_HIP, "place_spheres_noalloc: hipGetSymbolAddress(scratch): %s",
+                  hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return MPLBM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+mplbm_status mplbm_place_spheres_noalloc(int64_t lx, int64_t ly, int64_t lz, const float* spheres, int64_t count,
+                                         int64_t first_id, uint8_t* is_fluid_dev, int64_t* solid_number_dev) {
+  using namespace mplbm;
+  if (lx < 1 || ly < 1 || lz < 1 || lx > INT32_MAX || ly > INT32_MAX || lz > INT32_MAX)
+    return fail(nullptr, MPLBM_ERR_ARG, "place_spheres_noalloc: bad domain %ld x %ld x %ld", (long)lx, (long)ly,
+                (long)lz);
+  if (count < 0 || (count > 0 && !spheres) || !is_fluid_dev)
+    return fail(nullptr, MPLBM_ERR_ARG, "place_spheres_noalloc: null array or negative count");
+  const uint64_t n = (uint64_t)lx * (uint64_t)ly * (uint64_t)lz;
+  if ((uint64_t)count > kNoAllocMaxSpheres)
+    return fail(nullptr, MPLBM_ERR_ARG, "place_spheres_noalloc: %ld spheres exceed the scratch limit of %llu",
+                (long)count, (unsigned long long)kNoAllocMaxSpheres);
+  if (solid_number_dev && count > 0 && n > kNoAllocMaxCells)
+    return fail(nullptr, MPLBM_ERR_ARG, "place_spheres_noalloc: %llu cells exceed the scratch limit of %llu cells "
+                "with solid numbers", (unsigned long long)n, (unsigned long long)kNoAllocMaxCells);
+  if (const mplbm_status cs = check_spheres(spheres, count); cs != MPLBM_OK) return cs;
+  if (count == 0) return MPLBM_OK;
+  void* base = nullptr;
+  if (const mplbm_status sa = scratch_address(&base); sa != MPLBM_OK) return sa;
+  uint32_t* ids = solid_number_dev ? static_cast<uint32_t*>(base) : nullptr;
+  float4* sph = reinterpret_cast<float4*>(static_cast<unsigned char*>(base) + kNoAllocIdsBytes);
+  auto hip_fail = [](hipError_t e, const char* what) {
+    return fail(nullptr, MPLBM_ERR_HIP, "place_spheres_noalloc: %s: %s", what, hipGetErrorString(e));
+  };
+  hipError_t e = hipMemcpyAsync(sph, spheres, (size_t)count * sizeof(float4), hipMemcpyHostToDevice, 0);
+  if (e != hipSuccess) return hip_fail(e, "stage spheres");
+  // the sphere numbers of an earlier call are still in the scratch
+  if (ids) hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)std::min<uint64_t>((n + 255) / 256, 1u << 16)), dim3(256),
+                              0, 0, ids, n, 0u);
+  hipLaunchKernelGGL(k_voxelise, dim3((unsigned)std::min<int64_t>(count, 1 << 16)), dim3(kVoxThreads), 0, 0, sph,
+                     count, (int)lx, (int)ly, (int)lz, is_fluid_dev, ids);
+  if (ids)
+    hipLaunchKernelGGL(k_vox_numbers, dim3((unsigned)std::min<uint64_t>((n + 255) / 256, 1u << 16)), dim3(256), 0, 0,
+                       ids, solid_number_dev, first_id - 1, n);
+  if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "launch");
+  if ((e = hipStreamSynchronize(0)) != hipSuccess) return hip_fail(e, "synchronise");
+  return MPLBM_OK;
 }
 
 }  // extern "C"
