@@ -12,7 +12,12 @@ checks are the properties that hold at any size:
 * the FP64 run (bit-identical to the FP64 oracle on every small case)
   conserves mass to FP64 rounding;
 * FP32 against FP64: total mass drift, the liquid and vapour coexistence
-  densities (medians of the two phases) and the saturation.
+  densities (medians of the two phases) and the saturation.  These compare
+  two precisions on a chaotic porous run whose interfaces move apart, so
+  their bars are precision reports (1e-3 liquid, 1e-2 vapour, 0.02
+  saturation); the specification's 1e-6 agreement of equilibrium densities
+  with the oracle in the same precision after 10,000 steps is tested on a
+  flat interface in tests/test_gpu_configs.py.
 """
 import json
 import os
@@ -87,7 +92,7 @@ def test_mass_and_coexistence_against_fp64(packing, runs):
                       "liquid_median": [float(liq64), float(liq32)], "vapour_median": [float(vap64), float(vap32)],
                       "saturation": [sat64, sat32]}), flush=True)
     assert 0.45 < porosity < 0.55
-    assert abs(drift64) < 1e-11
+    assert abs(drift64) < 1e-12
     assert abs(drift32) < 1e-3
     # two phases separated: bulk liquid near 2 and vapour near 0.15 at G = -5
     assert liq64 > 1.6 and vap64 < 0.3
