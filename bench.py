@@ -51,7 +51,11 @@ def parse():
     ap.add_argument("--seg-x", type=int, default=0)
     ap.add_argument("--block-threads", type=int, default=0, help="compact kernel workgroup size (0, 256, 512)")
     ap.add_argument("--bounce-back", type=int, default=1, help="0: write-side (ghost) stores, 1: pull-side reads")
-    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=10, help="timed oracle steps per CPU baseline run (0: skip)")
+    ap.add_argument("--cpu-warmup", type=int, default=2, help="untimed oracle steps before the CPU baseline runs")
+    ap.add_argument("--cpu-repeats", type=int, default=3, help="CPU baseline runs; the median is reported")
+    ap.add_argument("--error-steps", type=int, default=100,
+                    help="steps of the error check of each storage form against FP64 (0: skip)")
     ap.add_argument("--profile-only", action="store_true", help="run only the FP32 timed loop (for rocprofv3)")
     ap.add_argument("--precision", action="store_true", help="error study of FP32/FP16S/BF16S against FP64")
     ap.add_argument("--precision-steps", type=int, default=1000)
@@ -146,21 +150,90 @@ def run_gpu(a, shape, fluid, rho, storage):
     }
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(a, shape, fluid, rho):
-    """The oracle (C, OpenMP, reference float arithmetic) on the same 512^3
-    workload for a bounded number of steps."""
+    """The oracle (C, OpenMP, the reference's float arithmetic in its 3-sweep
+    AoS structure: stream, boundary, collide) on the same 512^3 workload:
+    cpu_warmup untimed steps, then cpu_repeats runs of cpu_steps timed steps;
+    the median run is reported."""
     from oracle import oracle
     P = oracle.Params(*shape, **PHYS)
     st = oracle.State(P, fluid, rho, dtype=np.float32)
-    t = time.perf_counter()
-    st.run(a.cpu_steps)
-    dt = time.perf_counter() - t
+    st.run(a.cpu_warmup)
     nf = int(fluid.sum())
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    runs = []
+    for _ in range(a.cpu_repeats):
+        t = time.perf_counter()
+        st.run(a.cpu_steps)
+        runs.append(time.perf_counter() - t)
     del st
-    return {"value": nf * a.cpu_steps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"oracle (float, 3-sweep AoS restatement of the reference kernels) {a.cpu_steps} steps "
-                      f"of the same {shape[0]}^3 packing, {dt:.1f}s"}
+    dt = sorted(runs)[len(runs) // 2]
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(nf * a.cpu_steps / dt / 1e6, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
+            "sample": f"reference algorithm on CPU (restated): the oracle, {threads} OpenMP threads, "
+                      f"{a.cpu_warmup} warm-up steps then {a.cpu_steps} timed steps of the same "
+                      f"{shape[0]}^3 packing, median of {a.cpu_repeats} runs "
+                      f"({', '.join(f'{r:.2f}' for r in runs)} s)"}
+
+
+def error_check(a, shape, fluid, rho, storages):
+    """Error of each storage form against the FP64 path (bit-identical to the
+    FP64 oracle on every parity case) from the same start: density relative
+    L2 over fluid nodes after error_steps steps of the G = -5 run on the
+    512^3 packing, total mass drift (FP64 sums of the stored populations),
+    and the Darcy permeability (G = 0, tau = 0.8, g = 1e-6 along x, 600
+    steps) on the same packing generator at 64^3 with radii / 4."""
+    import multiphase_lbm_amd as mp
+    from multiphase_lbm_amd import harness as H
+    fl = fluid.astype(bool)
+    m0 = float(rho.astype(np.float64)[fl].sum())
+    steps = a.error_steps
+
+    def run(storage, rho0):
+        with mp.Context(*shape, storage=storage, kernel=a.kernel, **PHYS) as ctx:
+            ctx.set_geometry(fluid)
+            ctx.init_fields(rho0)
+            ctx.step(0, steps, write_macro=True)
+            r, _, _ = ctx.read_fields(np.float64)
+            return r, ctx.reduce()["mass"]
+
+    ref, _ = run("fp64", rho.astype(np.float64))
+    pshape = (64, 64, 64)
+    pfl, _ = H.generate_packing(pshape, a.seed, 10 ** 6, a.rmin / 4, a.rmax / 4, a.porosity)
+    tau, g = 0.8, 1e-6
+
+    def darcy(storage):
+        with mp.Context(*pshape, storage=storage, kernel=a.kernel, tau=tau, G=0.0, rhos=1.0,
+                        body_force=(g, 0.0, 0.0)) as ctx:
+            ctx.set_geometry(pfl)
+            ctx.init_fields(np.ones(pshape, np.float64 if storage == "fp64" else np.float32))
+            ctx.step(0, 600, write_macro=True)
+            _, _, u = ctx.read_fields(np.float64)
+        ux = np.where(pfl.astype(bool), u[..., 0] - (tau - 0.5) * g, 0.0)
+        return (tau - 0.5) / 3 * float(ux.mean()) / g
+
+    k64 = darcy("fp64")
+    out = {}
+    for s in storages:
+        r, mass = run(s, rho)
+        k = darcy(s)
+        out[s] = {"density_rel_l2": float(np.linalg.norm(r[fl] - ref[fl]) / np.linalg.norm(ref[fl])),
+                  "mass_rel_drift": mass / m0 - 1.0, "permeability_rel_err": abs(k / k64 - 1.0),
+                  "against": f"FP64 path, {steps} steps of the G=-5 run on this packing; permeability: 600 steps "
+                             f"at 64^3 (radii/4, porosity {a.porosity}), k_fp64 = {k64:.6f}"}
+        print(f"# error {s}: {out[s]}", file=sys.stderr, flush=True)
+    return out
 
 
 def traffic_from_profiles(storage, tag):
@@ -303,10 +376,12 @@ def ensemble_bench(a):
     """Batched ensemble: 64 seeded packings, cubic sizes n ~ U{48..160},
     porosity ~ U[0.35, 0.55], radii ~ U[2, 8], G ~ U[-5.5, -4.5] (seeds
     0..63), all advanced by one persistent launch per step (hipGraph replay),
-    FP32 and FP16 storage, workgroup size picked by the autotuner.  Prints one
-    JSON line with the aggregate MLUPS and per-sample saturation."""
+    FP32 and FP16 storage; the autotuner picks the workgroup size and, per
+    sample-size class, the tile and segment length.  Prints one JSON line
+    with the aggregate MLUPS, the autotuner's picks and per-sample
+    saturation."""
     from multiphase_lbm_amd import harness as H
-    from multiphase_lbm_amd.ensemble import Ensemble, autotune_block
+    from multiphase_lbm_amd.ensemble import SIZE_CLASSES, Ensemble, autotune, size_class
     rng = np.random.default_rng(2024)
     samples, fluids, rhos = [], [], []
     t = time.time()
@@ -336,8 +411,22 @@ def ensemble_bench(a):
                 e.set_geometry(k, fluids[k])
                 e.init_fields(k, rhos[k])
             return e
-        best, times = autotune_block(make)
+        shapes = [(q["lx"], q["ly"], q["lz"]) for q in samples]
+        t_tune = time.time()
+        best, picks, tuned_ms, trace = autotune(
+            make, shapes, log=lambda m: print(f"# autotune {storage}: {m}", file=sys.stderr, flush=True))
+        t_tune = time.time() - t_tune
+        default_ms = {b: ms for b, p, ms in trace if all(v == (0, 0, 0) for v in p.values())}
         with make(best) as ens:
+            for k, shp in enumerate(shapes):
+                if picks[size_class(shp)] != (0, 0, 0):
+                    ens.set_tiling(k, *picks[size_class(shp)])
+            tiles = {}
+            for cls in sorted(picks):
+                k = next(i for i, shp in enumerate(shapes) if size_class(shp) == cls)
+                ty, tz, sg = ens.tiling(k)
+                tiles[f"n{SIZE_CLASSES[cls][0]}-{min(SIZE_CLASSES[cls][1], 160)}"] = {
+                    "tile_y_z_seg": [ty, tz, sg], "interior_nodes_per_thread": round(ty * tz / best, 3)}
             ens.step(0, a.warmup)
             ms = ens.step_timed(a.warmup, a.steps) / a.steps
             done = a.warmup + a.steps
@@ -349,7 +438,9 @@ def ensemble_bench(a):
                 fl = fluids[k].astype(bool)
                 sat.append(round(float((r[fl] > 0.5 * (1.932442 + 0.156413)).mean()), 4))
             summ = ens.summary()
-        out[storage] = {"block_threads": best, "block_ms": {str(b): round(v, 4) for b, v in times.items()},
+        out[storage] = {"block_threads": best, "tiles": tiles,
+                        "untuned_ms_by_block": {str(b): round(v, 4) for b, v in default_ms.items()},
+                        "autotune": {"configurations_timed": len(trace), "seconds": round(t_tune, 1)},
                         "ms_per_step": round(ms, 4), "mlups": round(nf / ms / 1e3, 1),
                         "work_items": summ["work_items"], "saturation_step": done + rest, "saturation": sat}
         print(f"# ensemble {storage}: block {best}, {ms:.3f} ms/step, {nf / ms / 1e3:.1f} MLUPS", file=sys.stderr,
@@ -560,6 +651,7 @@ def main():
         print(f"# {s}: {json.dumps(res[s])}", file=sys.stderr, flush=True)
     if a.profile_only:
         return
+    errors = error_check(a, shape, fluid, rho, storages) if a.error_steps > 0 else {}
     cpu = cpu_baseline(a, shape, fluid, rho) if a.cpu_steps > 0 else None
     r = res["fp32"]
     line = {
@@ -585,12 +677,14 @@ def main():
                      "traffic": traffic_from_profiles("fp32", r["tag"]),
                      "bytes_per_update": r["bytes_per_update"], "kernel_ms": round(r["kernel_ms"], 4)},
         "cpu_baseline": cpu,
+        "error": errors.get("fp32"),
     }
     for s in storages:
         if s == "fp32":
             continue
         q = res[s]
         line[s] = {"value": round(q["mlups"], 1), "unit": "MLUPS", "vs_fp32": round(q["mlups"] / r["mlups"], 3),
+                   "error": errors.get(s),
                    "roofline": {"bound": "hbm", "achieved": round(q["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                                 "unit": "GB/s", "frac": round(q["achieved_gbs"] / HBM_PEAK_GBS, 4),
                                 "traffic": traffic_from_profiles(s, q["tag"]), "bytes_per_update": q["bytes_per_update"],

@@ -30,6 +30,15 @@ mplbm_status mplbm_ensemble_create(const mplbm_config* cfgs, int32_t n, int32_t 
 void mplbm_ensemble_destroy(mplbm_ensemble* ens);
 const char* mplbm_ensemble_last_error(const mplbm_ensemble* ens);
 
+/* Compact tile of one sample (sample = -1: every sample): tile_y x tile_z
+ * interior nodes, seg_x planes per work item (0 keeps the fitted value of
+ * that field), as mplbm_set_tiling; tile_y * tile_z must not exceed the
+ * workgroup size, which with it sets the interior nodes per thread.
+ * Results do not depend on the tiles; used by the ensemble autotuner
+ * (multiphase_lbm_amd/ensemble.py).  On failure the previous tiles stay. */
+mplbm_status mplbm_ensemble_set_tiling(mplbm_ensemble* ens, int32_t sample, int32_t tile_y, int32_t tile_z,
+                                       int32_t seg_x);
+
 /* Per-sample set-up, as mplbm_set_geometry / mplbm_init_fields. */
 mplbm_status mplbm_ensemble_set_geometry(mplbm_ensemble* ens, int32_t sample, const uint8_t* is_fluid);
 mplbm_status mplbm_ensemble_init_fields(mplbm_ensemble* ens, int32_t sample, const float* rho);

@@ -36,6 +36,7 @@
 #include <hip/hip_bf16.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -3063,6 +3064,9 @@ struct mplbm_ensemble {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int64_t fluid_nodes = 0;
+  // per-sample compact tile {tile_y, tile_z, seg_x} set by
+  // mplbm_ensemble_set_tiling; all zero = fitted to the block size
+  std::vector<std::array<int32_t, 3>> tile_req;
   std::string err;
 };
 
@@ -3106,13 +3110,24 @@ static mplbm_status ens_plan(mplbm_ensemble* e) {
     mplbm_ctx* c = e->ctx[s];
     if (e->compact) {
       // interior tile of at most `block` nodes, 32 wide in z where the axis
-      // allows; narrower in y until the LDS regions fit
-      const int tz = (int)std::min<int64_t>(32, c->cfg.lz);
-      int ty = (int)std::min<int64_t>(c->cfg.ly, std::max(1, std::min(e->block / tz, 16)));
-      const int seg = (int)std::min<int64_t>(32, c->cfg.lx);
+      // allows; narrower in y until the LDS regions fit.  Fields set by
+      // mplbm_ensemble_set_tiling replace the fitted ones (one attempt).
+      const std::array<int32_t, 3> r = s < e->tile_req.size() ? e->tile_req[s] : std::array<int32_t, 3>{0, 0, 0};
+      const bool asked = r[0] | r[1] | r[2];
+      const int tz = r[1] ? r[1] : (int)std::min<int64_t>(32, c->cfg.lz);
+      int ty = r[0] ? r[0] : (int)std::min<int64_t>(c->cfg.ly, std::max(1, std::min(e->block / tz, 16)));
+      const int seg = r[2] ? r[2] : (int)std::min<int64_t>(32, c->cfg.lx);
       mplbm_status st = MPLBM_ERR_ARG;
-      for (; ty >= 1; --ty)
-        if ((st = mplbm_set_tiling(c, ty, tz, seg)) == MPLBM_OK) break;
+      for (; ty >= 1; --ty) {
+        if (std::min<int64_t>(ty, c->cfg.ly) * std::min<int64_t>(tz, c->cfg.lz) > e->block) {
+          if (asked) {
+            st = fail(c, MPLBM_ERR_ARG, "tile %dx%d exceeds the %d threads of a workgroup", ty, tz, e->block);
+            break;
+          }
+          continue;
+        }
+        if ((st = mplbm_set_tiling(c, ty, tz, seg)) == MPLBM_OK || asked) break;
+      }
       if (st != MPLBM_OK) return ens_fail(e, st, "sample %zu: %s", s, c->err.c_str());
       e->lds_bytes = std::max(e->lds_bytes, csize == 8 ? compact_lds_bytes<double>(c->ctiling)
                                                        : compact_lds_bytes<float>(c->ctiling));
@@ -3340,6 +3355,27 @@ const char* mplbm_ensemble_last_error(const mplbm_ensemble* e) { return e ? e->e
     if (!(e)) return ens_fail(nullptr, MPLBM_ERR_ARG, "null ensemble");                   \
     if ((s) < 0 || (size_t)(s) >= (e)->ctx.size()) return ens_fail((e), MPLBM_ERR_ARG, "bad sample %d", (int)(s)); \
   } while (0)
+
+mplbm_status mplbm_ensemble_set_tiling(mplbm_ensemble* e, int32_t s, int32_t tile_y, int32_t tile_z,
+                                       int32_t seg_x) {
+  if (!e) return ens_fail(nullptr, MPLBM_ERR_ARG, "null ensemble");
+  if (s < -1 || s >= (int32_t)e->ctx.size()) return ens_fail(e, MPLBM_ERR_ARG, "bad sample %d", (int)s);
+  if (!e->compact) return ens_fail(e, MPLBM_ERR_UNSUPPORTED, "per-sample tiles apply to the compact ensemble");
+  if (tile_y < 0 || tile_z < 0 || seg_x < 0) return ens_fail(e, MPLBM_ERR_ARG, "negative tile");
+  if (hipStreamSynchronize(e->stream) != hipSuccess) return ens_fail(e, MPLBM_ERR_HIP, "sync");
+  const auto saved = e->tile_req;
+  e->tile_req.resize(e->ctx.size(), std::array<int32_t, 3>{0, 0, 0});
+  for (int32_t k = (s < 0 ? 0 : s); k < (s < 0 ? (int32_t)e->ctx.size() : s + 1); ++k)
+    e->tile_req[k] = {tile_y, tile_z, seg_x};
+  mplbm_status st = ens_plan(e);
+  if (st != MPLBM_OK) {
+    const std::string msg = e->err;
+    e->tile_req = saved;
+    (void)ens_plan(e);
+    return ens_fail(e, st, "%s", msg.c_str());
+  }
+  return MPLBM_OK;
+}
 
 mplbm_status mplbm_ensemble_set_geometry(mplbm_ensemble* e, int32_t s, const uint8_t* is_fluid) {
   ENS_SAMPLE(e, s);

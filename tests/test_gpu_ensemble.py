@@ -48,3 +48,32 @@ def test_ensemble_samples_match_oracle(block, use_graph, kernel):
             assert np.array_equal(f[fluid], st.fA[fluid].astype(np.float64)), f"sample {k}"
             assert np.array_equal(r[fluid], st.rho[fluid]), f"sample {k}"
         assert ens.summary()["work_items"] > 0
+
+
+def test_ensemble_per_sample_tiles_match_oracle():
+    """Tiles and segment lengths set per sample (the autotuner's knobs) leave
+    every sample bit-identical to its oracle run; a tile larger than the
+    workgroup is refused and the previous tiles stay."""
+    from multiphase_lbm_amd.ensemble import MplbmError
+    steps = 7
+    with Ensemble(SAMPLES, block_threads=256) as ens:
+        data = []
+        for k, s in enumerate(SAMPLES):
+            shape, fluid, rho = _inputs(k, s)
+            ens.set_geometry(k, fluid)
+            ens.init_fields(k, rho)
+            data.append((shape, fluid, rho))
+        ens.set_tiling(0, 4, 16, 3)
+        ens.set_tiling(1, 2, 64, 5)
+        ens.set_tiling(2, 0, 0, 2)   # fitted tile, 2-plane segments
+        assert ens.tiling(0) == (4, 16, 3) and ens.tiling(1)[2] == 5 and ens.tiling(2)[2] == 2
+        with pytest.raises(MplbmError, match="exceeds"):
+            ens.set_tiling(0, 16, 32, 8)   # 512 nodes > 256 threads
+        assert ens.tiling(0) == (4, 16, 3)
+        print("ensemble per-sample tiles", flush=True)
+        ens.step(0, steps, write_macro=True)
+        for k, (shape, fluid, rho) in enumerate(data):
+            s = SAMPLES[k]
+            P = oracle.Params(*shape, G=s["G"], tau=s["tau"], rhos=s["rhos"])
+            st = oracle.State(P, fluid, rho).run(steps)
+            assert np.array_equal(ens.read_populations(k)[fluid], st.fA[fluid].astype(np.float64)), f"sample {k}"
