@@ -432,25 +432,37 @@ struct Lattice {
   uint32_t ps;           // population sub-plane stride in elements (>= plane)
   uint32_t ncell;
   // compact layout (fluid nodes only): see cpix
-  const uint2* rw;       // [x][y][w]: {fluid bits of z = 32w..32w+31, plane rank of z = 32w}
+  const uint2* rw;       // [x][y][w][group]: see rank_word
   uint32_t W;            // words per (x, y) row
   int compact;
 };
 
 // Compact layout.  Only fluid nodes are stored.  The fluid nodes of an
-// x-plane are numbered 32-wide z block by block, rows of y within a block, z
-// within a row (their plane rank r; the rank word of (x, y, block) holds the
-// block's fluid bits of that row and the rank of its first node); population
-// i of the node of rank r in plane x lives at element ((x+1)*19 + i)*ps + r,
-// ps >= the largest plane count.  Memory planes 0 and lx+1 are copies of
-// planes lx-1 and 0 (kept current by the step kernel), so the three source
-// planes of a pull are always adjacent in memory.
+// x-plane are numbered (their plane rank r) 32-wide z block by block, and
+// within a block by blocks of rows; a block of rows numbers first the
+// columns 0-1 of its rows (row by row), then columns 2-29 (row by row, z
+// within a row), then columns 30-31, so the cells next to the neighbouring
+// z blocks form short contiguous ranges.  Each (x, y, z block) has three
+// rank words, one per column group: {fluid bits of the row's columns in the
+// group, rank of the group's first node in the row}.  Population i of the
+// node of rank r in plane x lives at element ((x+1)*19 + i)*ps + r, ps >= the
+// largest plane count.  Memory planes 0 and lx+1 are copies of planes lx-1
+// and 0 (kept current by the step kernel), so the three source planes of a
+// pull are always adjacent in memory.
+constexpr uint32_t kRankGroups = 3;
+__host__ __device__ __forceinline__ uint32_t rank_group(uint32_t c) {  // c = z mod 32
+  return c < 2 ? 1u : (c >= 30 ? 2u : 0u);
+}
+// index of the rank word of cell (y, z) within its plane's words
+__host__ __device__ __forceinline__ uint32_t rank_word(uint32_t W, uint32_t y, uint32_t z) {
+  return (y * W + (z >> 5)) * kRankGroups + rank_group(z & 31u);
+}
 __device__ __forceinline__ uint32_t plane_rank(const uint2* rw, uint32_t W, int ly, int x, int y, int z) {
-  const uint2 w = rw[((uint64_t)x * ly + y) * W + ((uint32_t)z >> 5)];
+  const uint2 w = rw[(uint64_t)x * ly * W * kRankGroups + rank_word(W, y, z)];
   return w.y + __popc(w.x & ((1u << (z & 31)) - 1u));
 }
 __device__ __forceinline__ bool rw_fluid(const uint2* rw, uint32_t W, int ly, int x, int y, int z) {
-  return (rw[((uint64_t)x * ly + y) * W + ((uint32_t)z >> 5)].x >> (z & 31)) & 1u;
+  return (rw[(uint64_t)x * ly * W * kRankGroups + rank_word(W, y, z)].x >> (z & 31)) & 1u;
 }
 template <class Mo>
 __device__ __forceinline__ uint64_t cpix(const Lattice<Mo>& L, int i, int mplane, uint32_t rank) {
@@ -979,7 +991,7 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
     const int r = (int)__umulhi((uint32_t)q, T.magic_rz);
     const int c = q - r * T.RZ;
     const uint32_t z = (uint32_t)zcrd[c];
-    qtab[q] = (((uint32_t)ycrd[r] * L.W + (z >> 5)) << 5) | (z & 31u);
+    qtab[q] = (rank_word(L.W, (uint32_t)ycrd[r], z) << 5) | (z & 31u);
     qpsi[q] = (r >= 1 && r <= T.TYi + 2 && c >= 1 && c <= T.TZi + 2) ? (r - 1) * T.PZ + (c - 1) : -1;
   }
   __syncthreads();
@@ -1013,7 +1025,7 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
     }
   };
   auto expand_issue = [&](int xp, uint2* w) {
-    const uint2* rwp = L.rw + (uint64_t)wrapx(xp) * L.ly * L.W;
+    const uint2* rwp = L.rw + (uint64_t)wrapx(xp) * L.ly * L.W * kRankGroups;
 #pragma unroll
     for (int m = 0; m < kPre; ++m) {
       const int q = tid + m * kThreads;
@@ -1026,7 +1038,7 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
       const int q = tid + m * kThreads;
       if (q < T.RYZ) put_rank(xp, q, w[m]);
     }
-    const uint2* rwp = L.rw + (uint64_t)wrapx(xp) * L.ly * L.W;
+    const uint2* rwp = L.rw + (uint64_t)wrapx(xp) * L.ly * L.W * kRankGroups;
     for (int q = tid + kPre * kThreads; q < T.RYZ; q += kThreads) put_rank(xp, q, rw_word(rwp, q));
   };
   auto expand = [&](int xp) {
@@ -1691,7 +1703,8 @@ struct mplbm_ctx {
   // compact layout
   bool compact = false;
   uint2* rw = nullptr;          // plane-rank words
-  uint32_t W = 0;               // words per row
+  uint32_t W = 0;               // 32-wide z blocks per row (three rank words each)
+  int rank_rows = 1;            // rows per block of the plane-rank order
   uint16_t* cpos = nullptr;     // work-list entries
   uint2* ctab = nullptr;        // [tile][x] list table
   size_t cpos_cap = 0, ctab_cap = 0;
@@ -1930,8 +1943,10 @@ static size_t compact_lds_bytes(const CTiling& T, size_t shift_bytes = 0) {
 
 // Work lists of the compact kernel: for every tile and plane, the psi-region
 // positions (r, c) of the fluid nodes the tile pulls, interior nodes first
-// (the ones it collides), then the ring.  The region covers the tile's
-// in-domain interior [1, ny] x [1, nz] and the ring of one node around it.
+// (the ones it collides), then the ring, each in plane-rank order so that a
+// wavefront's 64 entries address few memory windows.  The region covers the
+// tile's in-domain interior [1, ny] x [1, nz] and the ring of one node
+// around it.
 static mplbm_status build_lists(mplbm_ctx* ctx) {
   const CTiling& T = ctx->ctiling;
   const int64_t lx = ctx->cfg.lx, ly = ctx->cfg.ly, lz = ctx->cfg.lz;
@@ -1941,23 +1956,45 @@ static mplbm_status build_lists(mplbm_ctx* ctx) {
   pos.reserve((size_t)(ctx->fluid_nodes * 3 / 2 + 1024));
   const uint8_t* m = ctx->hmask.data();
   std::vector<uint16_t> ring;
+  // plane-rank order of a cell (the same in every plane): z block, block of
+  // rows, column group (0-1, 2-29, 30-31), row, column
+  const int64_t yb = ctx->rank_rows;
+  const int64_t nyb = (ly + yb - 1) / yb;
+  auto order_key = [&](int64_t y, int64_t z) {
+    const uint32_t c = (uint32_t)(z & 31);
+    const uint64_t g = c < 2 ? 0 : (c >= 30 ? 2 : 1);
+    return (((((uint64_t)(z >> 5) * nyb + (uint64_t)(y / yb)) * 3 + g) * (uint64_t)ly + (uint64_t)y) << 5) | c;
+  };
+  struct Cell {
+    uint64_t key;
+    uint16_t p;
+    uint8_t interior;
+    int64_t y, z;
+  };
+  std::vector<Cell> cells;
   for (int64_t t = 0; t < ntile; ++t) {
     const int64_t tyi = t / T.tiles_z, tzi = t - tyi * T.tiles_z;
     const int64_t y0 = tyi * T.TYi, z0 = tzi * T.TZi;
     const int ny = (int)std::min<int64_t>(T.TYi, ly - y0), nz = (int)std::min<int64_t>(T.TZi, lz - z0);
+    cells.clear();
+    for (int r = 0; r <= ny + 1; ++r) {
+      const int64_t y = ((y0 - 1 + r) % ly + ly) % ly;
+      for (int c = 0; c <= nz + 1; ++c) {
+        const int64_t z = ((z0 - 1 + c) % lz + lz) % lz;
+        const bool in = r >= 1 && r <= ny && c >= 1 && c <= nz;
+        cells.push_back(Cell{order_key(y, z), (uint16_t)(r * T.PZ + c), (uint8_t)in, y, z});
+      }
+    }
+    std::sort(cells.begin(), cells.end(),
+              [](const Cell& a, const Cell& b) { return a.key != b.key ? a.key < b.key : a.p < b.p; });
     for (int64_t x = 0; x < lx; ++x) {
       const size_t first = pos.size();
       ring.clear();
-      for (int r = 0; r <= ny + 1; ++r) {
-        const int64_t y = ((y0 - 1 + r) % ly + ly) % ly;
-        const uint8_t* row = m + ((size_t)x * ly + y) * lz;
-        for (int c = 0; c <= nz + 1; ++c) {
-          const int64_t z = ((z0 - 1 + c) % lz + lz) % lz;
-          if (!row[z]) continue;
-          const uint16_t p = (uint16_t)(r * T.PZ + c);
-          if (r >= 1 && r <= ny && c >= 1 && c <= nz) pos.push_back(p);
-          else ring.push_back(p);
-        }
+      const uint8_t* pl = m + (size_t)x * ly * lz;
+      for (const Cell& e : cells) {
+        if (!pl[(size_t)e.y * lz + e.z]) continue;
+        if (e.interior) pos.push_back(e.p);
+        else ring.push_back(e.p);
       }
       const size_t nint = pos.size() - first;
       pos.insert(pos.end(), ring.begin(), ring.end());
@@ -1989,34 +2026,52 @@ constexpr uint32_t kRankAlign = 64;  // ranks: one wavefront of nodes
 static mplbm_status build_compact(mplbm_ctx* ctx) {
   const int64_t lx = ctx->cfg.lx, ly = ctx->cfg.ly, lz = ctx->cfg.lz;
   const uint32_t W = (uint32_t)((lz + 31) / 32);
-  std::vector<uint2> rw((size_t)(lx * ly * W));
+  std::vector<uint2> rw((size_t)(lx * ly * W) * kRankGroups);
   uint64_t maxc = 0;
   const uint8_t* m = ctx->hmask.data();
-  // plane rank order: 32-wide z blocks outermost, then y, then z, so the rows
-  // of a tile 32 nodes wide form one contiguous range of a plane.  Each
-  // tile's range starts at a multiple of 64 ranks (the ranks between are
-  // never addressed): a wavefront takes 64 consecutive interior nodes, so
-  // its stores and its in-plane pulls cover whole aligned windows (512^3
-  // FP32 3.50 -> 3.31 ms, FP16S 2.17 -> 2.13 ms, for ~14% more lattice
-  // memory).  Tiles chosen later keep the layout; only alignment is lost.
-  const int64_t ty_block = std::max(1, ctx->ctiling.TYi);
+  // plane rank order: 32-wide z blocks outermost, then blocks of rows as
+  // high as the tile, so a tile 32 nodes wide is one contiguous range of a
+  // plane.  Each range starts at a multiple of 64 ranks (the ranks between
+  // are never addressed): a wavefront takes 64 consecutive interior nodes,
+  // so its stores and its in-plane pulls cover whole aligned windows (512^3
+  // FP32 3.50 -> 3.31 ms, FP16S 2.17 -> 2.13 ms, for ~12% more lattice
+  // memory).  Within a range, columns 0-1 of its rows come first, then
+  // columns 2-29, then 30-31: the neighbouring tiles' ring nodes and the
+  // cross-block pulls of the edge columns read a few short ranges instead of
+  // one line per row.  Tiles chosen later keep the layout (and are as fast
+  // when their rows stay within these blocks).
+  const int64_t yb = std::max(1, ctx->ctiling.TYi);
+  if ((uint64_t)ly * W * kRankGroups >= (1ull << 27)) return fail(ctx, MPLBM_ERR_ARG, "planes too large for the compact layout");
+  const uint32_t gmask[kRankGroups] = {0x3FFFFFFCu, 0x00000003u, 0xC0000000u};  // columns 2-29, 0-1, 30-31
+  const uint32_t gorder[kRankGroups] = {1, 0, 2};                                // numbering order of the groups
+  std::vector<uint32_t> rowbits((size_t)ly);
   for (int64_t x = 0; x < lx; ++x) {
     uint64_t r = 0;  // exceeds 32 bits only for planes the size check below refuses
     for (uint32_t w = 0; w < W; ++w) {
       for (int64_t y = 0; y < ly; ++y) {
-        if (y % ty_block == 0) r = (r + kRankAlign - 1) / kRankAlign * kRankAlign;
         const uint8_t* row = m + ((size_t)x * ly + y) * lz;
         uint32_t bits = 0;
         for (int b = 0; b < 32; ++b) {
           const int64_t z = (int64_t)w * 32 + b;
           if (z < lz && row[z]) bits |= 1u << b;
         }
-        rw[(size_t)((x * ly + y) * W + w)] = make_uint2(bits, (uint32_t)r);
-        r += (uint64_t)__builtin_popcount(bits);
+        rowbits[(size_t)y] = bits;
+      }
+      for (int64_t y0 = 0; y0 < ly; y0 += yb) {
+        r = (r + kRankAlign - 1) / kRankAlign * kRankAlign;
+        const int64_t y1 = std::min(ly, y0 + yb);
+        for (uint32_t g : gorder) {
+          for (int64_t y = y0; y < y1; ++y) {
+            const uint32_t b = rowbits[(size_t)y] & gmask[g];
+            rw[((size_t)(x * ly + y) * W + w) * kRankGroups + g] = make_uint2(b, (uint32_t)r);
+            r += (uint64_t)__builtin_popcount(b);
+          }
+        }
       }
     }
     maxc = std::max<uint64_t>(maxc, r);
   }
+  ctx->rank_rows = (int)yb;
   uint64_t ps = std::max<uint64_t>(64, (maxc + 63) / 64 * 64);
   int64_t pad = ctx->cfg.plane_pad;
   if (pad < 0) pad = (ps % 4096 == 0) ? 64 : 0;
