@@ -967,7 +967,8 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
   int* zcrd = ycrd + (T.TYi + 4);                                            // TZi + 4 columns
   uint32_t* qtab = reinterpret_cast<uint32_t*>(zcrd + (T.TZi + 4));          // RYZ: rank word << 5 | bit
   int* qpsi = reinterpret_cast<int*>(qtab + T.RYZ);                          // RYZ: psi index or -1
-  typename Mo::S* shr = reinterpret_cast<typename Mo::S*>(qpsi + T.RYZ);     // kSlots x RYZ node shifts (kNS)
+  int* pent = qpsi + T.RYZ;                                                  // kThreads: next list entry
+  typename Mo::S* shr = reinterpret_cast<typename Mo::S*>(pent + kThreads);  // kSlots x RYZ node shifts (kNS)
 
   const int tyi = tile / T.tiles_z;
   const int tzi = tile - tyi * T.tiles_z;
@@ -1160,14 +1161,34 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
   uint32_t ownN = 0;
   bool hasN = false;
 
-  auto psi_plane = [&](int xp, bool keep) {
+  // Work of plane xp: the tile's list entries and this thread's first entry,
+  // fetched one plane ahead of its pull so that the entry's load completes
+  // with the previous plane's population loads instead of adding a memory
+  // round trip before the pull's own address computation.
+  struct PlaneWork {
+    uint32_t first;
+    int nint, npull;
+  };
+  auto plane_work = [&](int xp) -> PlaneWork {
     // uniform index: a scalar load through the constant address space
     using CU64 = const __attribute__((address_space(4))) unsigned long long;
     const unsigned long long tw = ((CU64*)W.tab)[(uint64_t)tile * L.lx + wrapx(xp)];
-    const uint2 tb = make_uint2((uint32_t)tw, (uint32_t)(tw >> 32));
-    const uint32_t first = __builtin_amdgcn_readfirstlane(tb.x);
-    const int nint = (int)__builtin_amdgcn_readfirstlane(tb.y & 0xFFFFu);
-    const int npull = (int)__builtin_amdgcn_readfirstlane(tb.y >> 16);
+    PlaneWork w;
+    w.first = __builtin_amdgcn_readfirstlane((uint32_t)tw);
+    w.nint = (int)__builtin_amdgcn_readfirstlane((uint32_t)(tw >> 32) & 0xFFFFu);
+    w.npull = (int)__builtin_amdgcn_readfirstlane((uint32_t)(tw >> 48));
+    return w;
+  };
+  // this thread's list entry (entry 0 of the plane when it has none: the
+  // list always holds one more entry than the last plane uses)
+  auto entry_of = [&](const PlaneWork& w) -> int {
+    return (int)W.pos[w.first + (tid < w.npull ? (uint32_t)tid : 0u)];
+  };
+
+  auto psi_plane = [&](int xp, bool keep, const PlaneWork& w, int pe) {
+    const uint32_t first = w.first;
+    const int nint = w.nint;
+    const int npull = w.npull;
     C* pz = psl + slot(xp) * T.PYZ;
     for (int k = tid + kThreads; k < npull; k += kThreads) {
       const int p = W.pos[first + k];
@@ -1180,7 +1201,8 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
     }
     hasN = false;
     if (tid < npull) {
-      pN = W.pos[first + tid];
+      if constexpr (sizeof(S) == 4) pN = pe;  // entry loaded one plane ahead
+      else pN = W.pos[first + tid];
       int gy, gz;
       S raw[kQ];
       C f[kQ];
@@ -1201,7 +1223,9 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
     }
   };
 
-  // collision of the fluid interior node at psi position p of plane x
+  // collision of the fluid interior node at psi position p of plane x (rank
+  // offset own): force, velocity shift, BGK, encode, stores (with the ghost
+  // copies of the first and last planes); macro fields on macro steps
   auto collide_node = [&](int x, int p, C* f, C rho, int gy, int gz, uint32_t own) {
     const int xw = wrapx(x);
     const C* pzm = psl + slot(x - 1) * T.PYZ;
@@ -1224,9 +1248,6 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
     }
     C u[3];
     collide<Mo>(f, rho, F, ph, u);
-    const Rsrc O = make_rsrc(L.fout + (uint64_t)(xw + 1) * kQ * L.ps);
-    uint32_t psb_o = psb;
-    asm volatile("" : "+s"(psb_o));
     S e[kQ];
     if constexpr (kSR) {
       // dither stream of this node and step: a mixed seed, then a linear
@@ -1254,6 +1275,9 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
 #pragma unroll
       for (int i = 0; i < kQ; ++i) e[i] = encode<Mo>(f[i], i, ph);
     }
+    const Rsrc O = make_rsrc(L.fout + (uint64_t)(xw + 1) * kQ * L.ps);
+    uint32_t psb_o = psb;
+    asm volatile("" : "+s"(psb_o));
 #pragma unroll
     for (int i = 0; i < kQ; ++i) bstore<S, compact_store_policy<S>()>(e[i], O, own, (uint32_t)i * psb_o);
     if (xw == 0 || xw == L.lx - 1) {  // ghost copies of the first and last planes
@@ -1281,14 +1305,40 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
   // prologue: rank regions of planes x0-2 .. x0+2, psi of planes x0-1, x0
   for (int k = -2; k <= 2; ++k) expand(x0 + k);
   __syncthreads();
-  psi_plane(x0 - 1, false);
-  psi_plane(x0, true);
+  {
+    const PlaneWork w = plane_work(x0 - 1);
+    psi_plane(x0 - 1, false, w, entry_of(w));
+  }
+  {
+    const PlaneWork w = plane_work(x0);
+    psi_plane(x0, true, w, entry_of(w));
+  }
+  // With 4-byte storage the list entry of the next plane's pull is
+  // loaded one plane ahead, before this plane's population loads, and kept
+  // in this thread's LDS word (written after the pull has waited for its
+  // loads), so no wait on it precedes the pull's address computation: such
+  // a wait (waiting for the entry alone, 512^3 FP32 3.18 -> 3.06 ms) also
+  // waits for the stores of the previous collision, as loads and stores
+  // share one counter.  The 16-bit kernels load the entry in the pull: they
+  // run at 80 registers, and one more held across the pull splits its 19
+  // loads into two batches (FP16S 2.11 -> 2.18 ms); FP64 would drop from
+  // three waves per SIMD to two.
+  constexpr bool kAhead = sizeof(S) == 4;  // as in psi_plane
+  PlaneWork wc = plane_work(x0 + 1);
+  if constexpr (kAhead) pent[tid] = entry_of(wc);
 
   for (int j = 0; j < n; ++j) {
     const int x = __builtin_amdgcn_readfirstlane(x0 + j);
     const bool more = j + 2 <= n;  // rank regions are needed up to plane x1 + 1
     uint2 wpre[kPre];
     if (more) expand_issue(x + 3, wpre);
+    // list entries of plane x+2, pulled in the next iteration
+    PlaneWork wn{};
+    int en = 0;
+    if (more) {
+      wn = plane_work(x + 2);
+      if constexpr (kAhead) en = entry_of(wn);
+    }
     // this thread's interior node of plane x, pulled one iteration ago
     NT fk[kNW];
 #pragma unroll
@@ -1297,7 +1347,13 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
     const int p = pN;
     const uint32_t own = ownN;
     const bool has = hasN;
-    psi_plane(x + 1, true);
+    if constexpr (kAhead) {
+      psi_plane(x + 1, true, wc, pent[tid]);
+      pent[tid] = en;  // each thread reads back only its own word
+    } else {
+      psi_plane(x + 1, true, wc, 0);
+    }
+    wc = wn;
     if (more) expand_finish(x + 3, wpre);
     __syncthreads();
 
@@ -1332,13 +1388,13 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
 // waves per SIMD (80 VGPRs), three 512-thread workgroups per CU like the
 // global-shift kernel (they would take 81 and drop to two); 1 leaves the
 // register budget to the compiler (the face variants would spill at 80).
-template <bool kNS, bool kFaces>
+template <class Mo, bool kNS, bool kFaces>
 constexpr int compact_min_waves() {
   return kNS && !kFaces ? 6 : 1;
 }
 
 template <class Mo, bool kFaces, int kThreads, bool kSR = false, bool kNS = false>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(compact_min_waves<kNS, kFaces>())))
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(compact_min_waves<Mo, kNS, kFaces>())))
 void k_compact(Lattice<Mo> L, Phys<typename Mo::C> ph, CTiling T, CLists W) {
   extern __shared__ unsigned char smem_raw[];
   int b = blockIdx.x;
@@ -1893,7 +1949,8 @@ static mplbm_status choose_ctiling(mplbm_ctx* ctx) {
   {
     const size_t csz = c.storage == MPLBM_FP64 ? 8 : 4;
     const size_t lds = kCompactSlots * ((size_t)T.PYZ * csz + (size_t)T.RYZ * 4) + (size_t)(ty + 4 + tz + 4) * 4 +
-                       2 * (size_t)T.RYZ * 4 + (c.shift == MPLBM_SHIFT_NODE ? kCompactSlots * (size_t)T.RYZ * 2 : 0);
+                       2 * (size_t)T.RYZ * 4 + 1024 * 4 +
+                       (c.shift == MPLBM_SHIFT_NODE ? kCompactSlots * (size_t)T.RYZ * 2 : 0);
     if (lds > kMaxGroupLds)
       return fail(ctx, MPLBM_ERR_ARG, "tile %dx%d needs %zu bytes of LDS (max %zu)", ty, tz, lds, kMaxGroupLds);
   }
@@ -1934,11 +1991,12 @@ static mplbm_status choose_ctiling(mplbm_ctx* ctx) {
   return MPLBM_OK;
 }
 
-// shift_bytes: bytes of a node shift in the node-shifted form, else 0
+// shift_bytes: bytes of a node shift in the node-shifted form, else 0;
+// threads: the workgroup size of the launch (one list-entry word per thread)
 template <class C>
-static size_t compact_lds_bytes(const CTiling& T, size_t shift_bytes = 0) {
+static size_t compact_lds_bytes(const CTiling& T, size_t shift_bytes, int threads) {
   return kCompactSlots * ((size_t)T.PYZ * sizeof(C) + (size_t)T.RYZ * 4) + (size_t)(T.TYi + 4 + T.TZi + 4) * 4 +
-         2 * (size_t)T.RYZ * 4 + kCompactSlots * (size_t)T.RYZ * shift_bytes;
+         2 * (size_t)T.RYZ * 4 + (size_t)threads * 4 + kCompactSlots * (size_t)T.RYZ * shift_bytes;
 }
 
 // Work lists of the compact kernel: for every tile and plane, the psi-region
@@ -2116,7 +2174,7 @@ static mplbm_status launch_step(mplbm_ctx* ctx, int64_t step, int32_t reverse, u
   if (ctx->kernel == MPLBM_KERNEL_COMPACT) {
     const CTiling& T = ctx->ctiling;
     const CLists WL{ctx->cpos, ctx->ctab};
-    const size_t shm = compact_lds_bytes<C>(T, ph.node_shift ? sizeof(typename Mo::S) : 0);
+    const size_t shm = compact_lds_bytes<C>(T, ph.node_shift ? sizeof(typename Mo::S) : 0, T.threads);
     // the workgroup size is at least the tile's interior count (one kept
     // interior node per thread); 1024 only for non-FP64 storage
     // stochastic rounding and node shifts are separate instantiations so the
@@ -3192,8 +3250,8 @@ static mplbm_status ens_plan(mplbm_ensemble* e) {
         if ((st = mplbm_set_tiling(c, ty, tz, seg)) == MPLBM_OK || asked) break;
       }
       if (st != MPLBM_OK) return ens_fail(e, st, "sample %zu: %s", s, c->err.c_str());
-      e->lds_bytes = std::max(e->lds_bytes, csize == 8 ? compact_lds_bytes<double>(c->ctiling)
-                                                       : compact_lds_bytes<float>(c->ctiling));
+      e->lds_bytes = std::max(e->lds_bytes, csize == 8 ? compact_lds_bytes<double>(c->ctiling, 0, e->block)
+                                                       : compact_lds_bytes<float>(c->ctiling, 0, e->block));
       continue;
     }
     int ty, tz, seg;
