@@ -120,11 +120,13 @@ def test_fused_tilings_agree(tiles, bb):
 
 
 @pytest.mark.parametrize("tiles", [(1, 1, 1, 0), (3, 5, 2, 0), (8, 32, 3, 0), (0, 0, 0, 0), (2, 19, 9, 512),
-                                   (23, 7, 4, 0), (16, 32, 5, 0), (23, 19, 9, 1024)])
+                                   (23, 7, 4, 0), (16, 32, 5, 0), (23, 19, 9, 1024), (16, 16, 4, 256)])
 def test_compact_tilings_agree(tiles):
     """Compact kernel: single-node tiles, partial tiles in y and z, tiles
     wider than the domain, short and whole-axis segments, workgroups of 256,
-    512 and 1024 threads."""
+    512 and 1024 threads; a 16 x 16 tile with 256 threads, whose psi region
+    (324 positions, mostly fluid here) holds more pulls than threads, so the
+    extra ring entries take the second pass over the list."""
     ty, tz, seg, nt = tiles
     shape = (9, 23, 19)
     fluid, rho = _case(shape, 4, 1.5, 3.5, seed=7)
