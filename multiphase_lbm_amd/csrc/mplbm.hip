@@ -3150,9 +3150,18 @@ struct EnsSampleC {
 };
 
 // Compact form of the persistent launch: each item is a compact march of
-// one sample's tile and x segment.
+// one sample's tile and x segment.  The 16-bit forms are held to 80 VGPRs
+// (6 waves per SIMD, as the single-domain kernel): left to itself the
+// compiler takes 88 (5 waves), and the 12 bytes per thread it spills at 80
+// cost less than the wave gains (64 packings: FP16S 1.64 -> 1.53 ms per
+// step; FP32 FAST held to 96 was slower, 1.695 -> 1.709 ms).
+template <class Mo>
+constexpr int ensemble_min_waves() {
+  return sizeof(typename Mo::S) == 2 ? 6 : 1;
+}
 template <class Mo, int kBlock>
-__global__ __launch_bounds__(kBlock) void k_ensemble_c(const EnsItem* __restrict__ items, int nitems,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ensemble_min_waves<Mo>())))
+void k_ensemble_c(const EnsItem* __restrict__ items, int nitems,
                                                        const EnsSampleC<Mo>* __restrict__ samples, int parity,
                                                        int macro, EnsQueue* queue) {
   extern __shared__ unsigned char smem_raw[];
