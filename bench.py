@@ -56,7 +56,9 @@ def parse():
     ap.add_argument("--cpu-repeats", type=int, default=3, help="CPU baseline runs; the median is reported")
     ap.add_argument("--error-steps", type=int, default=100,
                     help="steps of the error check of each storage form against FP64 (0: skip)")
-    ap.add_argument("--profile-only", action="store_true", help="run only the FP32 timed loop (for rocprofv3)")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="run only the timed loop of the first storage form (with --ensemble: of each form, fitted "
+                         "tiles, no autotuner, no saturation run), for rocprofv3")
     ap.add_argument("--precision", action="store_true", help="error study of FP32/FP16S/BF16S against FP64")
     ap.add_argument("--precision-steps", type=int, default=1000)
     ap.add_argument("--slab-steps", type=int, default=3000)
@@ -412,6 +414,15 @@ def ensemble_bench(a):
                 e.init_fields(k, rhos[k])
             return e
         shapes = [(q["lx"], q["ly"], q["lz"]) for q in samples]
+        if a.profile_only:
+            # fitted tiles at one workgroup size, timed steps only (for rocprofv3)
+            block = a.block_threads or (256 if storage in ("fp16s", "bf16s") else 512)
+            with make(block) as ens:
+                ens.step(0, a.warmup)
+                ms = ens.step_timed(a.warmup, a.steps) / a.steps
+            print(f"# ensemble {storage}: block {block}, fitted tiles, {ms:.3f} ms/step, {nf / ms / 1e3:.1f} MLUPS",
+                  file=sys.stderr, flush=True)
+            continue
         t_tune = time.time()
         best, picks, tuned_ms, trace = autotune(
             make, shapes, log=lambda m: print(f"# autotune {storage}: {m}", file=sys.stderr, flush=True))
@@ -445,7 +456,8 @@ def ensemble_bench(a):
                         "work_items": summ["work_items"], "saturation_step": done + rest, "saturation": sat}
         print(f"# ensemble {storage}: block {best}, {ms:.3f} ms/step, {nf / ms / 1e3:.1f} MLUPS", file=sys.stderr,
               flush=True)
-    print(json.dumps(out), flush=True)
+    if not a.profile_only:
+        print(json.dumps(out), flush=True)
 
 
 def drainage_fields(shape, seed, porosity):
