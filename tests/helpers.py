@@ -2,11 +2,41 @@
 import numpy as np
 
 
-def random_spheres(shape, n, rmin, rmax, seed):
+def random_spheres(shape, n, rmin, rmax, seed, nonoverlap=False, max_tries=100000):
+    """n spheres (x, y, z, r), centres uniform in the domain, radii uniform
+    in [rmin, rmax].  nonoverlap: seeded random sequential addition, each
+    candidate kept only if its periodic (minimum-image) centre distance to
+    every kept sphere is at least the sum of the radii."""
     rng = np.random.default_rng(seed)
-    c = rng.uniform(0, 1, size=(n, 3)) * np.array(shape)
-    r = rng.uniform(rmin, rmax, size=n)
-    return np.concatenate([c, r[:, None]], axis=1)
+    L = np.array(shape, float)
+    if not nonoverlap:
+        c = rng.uniform(0, 1, size=(n, 3)) * L
+        r = rng.uniform(rmin, rmax, size=n)
+        return np.concatenate([c, r[:, None]], axis=1)
+    kept = []
+    for _ in range(max_tries):
+        if len(kept) == n:
+            break
+        c = rng.uniform(0, 1, size=3) * L
+        r = rng.uniform(rmin, rmax)
+        if kept:
+            k = np.array(kept)
+            d = np.abs(k[:, :3] - c)
+            d = np.minimum(d, L - d)
+            if np.any(np.sqrt((d * d).sum(1)) < k[:, 3] + r):
+                continue
+        kept.append([c[0], c[1], c[2], r])
+    if len(kept) < n:
+        raise RuntimeError(f"placed {len(kept)} of {n} non-overlapping spheres")
+    return np.array(kept)
+
+
+def config1_case(seed=1):
+    """The 64^3 workload: 40 non-overlapping spheres, radii uniform in [3, 6],
+    seeded random sequential addition; rho0 = 0.7 with +/-0.01 noise."""
+    shape = (64, 64, 64)
+    fluid = sphere_mask(shape, random_spheres(shape, 40, 3.0, 6.0, seed, nonoverlap=True))
+    return shape, fluid, noisy_rho(shape, 0.7, 0.01, seed + 1)
 
 
 def sphere_mask(shape, spheres):

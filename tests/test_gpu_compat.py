@@ -9,7 +9,8 @@ MultiphaseLBM::run_gpu (src/multiphase_lbm.cu:584-606):
                                  &dcollision_f, &dstream_f, &dforce);
     for (; step < step_max; step++) multiphaselbm_iteration(step, statPeriod, reverse, inject_type);
 
-64^3 with 40 spheres, G = -5, injection every 25 steps on y < ly/2.  On each
+64^3 with 40 non-overlapping spheres (the 64^3 workload), G = -5,
+injection every 25 steps on y < ly/2.  On each
 statistics step the caller's density and pressure arrays hold the oracle's
 fields of that step bit for bit (FP32, the reference's precision).  The
 caller's arrays are host memory or managed memory (the reference allocates
@@ -20,7 +21,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle
-from tests.helpers import noisy_rho, random_spheres, sphere_mask
+from tests.helpers import config1_case
 
 pytestmark = pytest.mark.gpu
 
@@ -64,9 +65,7 @@ class _Managed:
 
 @pytest.mark.parametrize("memory", ["host", "managed"])
 def test_reference_call_sequence_matches_oracle(memory):
-    shape = (64, 64, 64)
-    fluid = sphere_mask(shape, random_spheres(shape, 40, 3.0, 6.0, 1))
-    rho0 = noisy_rho(shape, 0.7, 0.01, 2)
+    shape, fluid, rho0 = config1_case()
     print(f"case compat {memory} 64^3", flush=True)
     st = oracle.State(oracle.Params(*shape, **PARAMS), fluid, rho0, dtype=np.float32)
     p_init = st.p.copy()

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle
-from tests.helpers import DENSITY_L2_BAR, noisy_rho, random_spheres, sphere_mask
+from tests.helpers import DENSITY_L2_BAR, config1_case, noisy_rho, random_spheres, sphere_mask
 
 pytestmark = pytest.mark.gpu
 
@@ -78,9 +78,9 @@ def test_fp32_small_bit_exact(kernel):
 
 @pytest.mark.parametrize("kernel,bb", [("fused", 0), ("fused", 1), ("twopass", 0), ("compact", 1)])
 def test_fp32_config1_100_steps(kernel, bb):
-    """64^3, 40 spheres r in [3,6], G=-5, tau=1, rho0=0.7 +/- 0.01, 100 steps."""
-    shape = (64, 64, 64)
-    fluid, rho = _case(shape, 40, 3.0, 6.0, seed=1)
+    """64^3, 40 non-overlapping spheres r in [3,6] (random sequential
+    addition), G=-5, tau=1, rho0=0.7 +/- 0.01, 100 steps."""
+    shape, fluid, rho = config1_case()
     print(f"case fp32 config1 {kernel} bb {bb}", flush=True)
     st, (g_rho, g_p, g_u, g_f, stats) = _run_pair(shape, fluid, rho, 100, "fp32", kernel, bounce_back=bb, **SC)
     assert _relerr(g_rho[fluid], st.rho[fluid]) <= 1e-5
@@ -92,8 +92,7 @@ def test_fp32_config1_100_steps(kernel, bb):
 
 @pytest.mark.parametrize("kernel", ["fused", "compact"])
 def test_fp64_config1_100_steps(kernel):
-    shape = (64, 64, 64)
-    fluid, rho = _case(shape, 40, 3.0, 6.0, seed=1)
+    shape, fluid, rho = config1_case()
     print(f"case fp64 config1 {kernel}", flush=True)
     st, (g_rho, g_p, g_u, g_f, stats) = _run_pair(shape, fluid, rho, 100, "fp64", kernel, **SC)
     assert _relerr(g_rho[fluid], st.rho[fluid]) <= 1e-12
@@ -293,8 +292,7 @@ def test_fp32_fast_math_error_vs_oracle():
     bar of the FP32 form (measured ~2.4e-4); the reference-arithmetic FP32
     form is the one that does (bit-identical).  Asserted: FAST stays within
     1e-3, the bar of the 16-bit storage forms."""
-    shape = (64, 64, 64)
-    fluid, rho = _case(shape, 40, 3.0, 6.0, seed=1)
+    shape, fluid, rho = config1_case()
     print("case fp32 fast config1", flush=True)
     st, (g_rho, g_p, g_u, g_f, stats) = _run_pair(shape, fluid, rho, 100, "fp32fast", "fused", **SC)
     rel_rho = _relerr(g_rho[fluid], st.rho[fluid])

@@ -399,3 +399,24 @@ def test_erosion_against_brute_force(tmp_path, layers_wanted):
     assert np.all(fluid.astype(bool) >= fluid0)  # erosion only turns solids fluid
     # the density initialisation is untouched by the erosion
     assert np.array_equal(rho, H.rho_init(fluid.shape, 0.7, 0.0285714))
+
+
+def test_random_sequential_addition_spheres_do_not_overlap():
+    """The 64^3 workload's geometry: 40 spheres, radii in [3, 6], placed by
+    seeded random sequential addition, pairwise periodic centre distance at
+    least the sum of the radii; the same seed gives the same spheres."""
+    from tests.helpers import config1_case, random_spheres
+    shape = (64, 64, 64)
+    sph = random_spheres(shape, 40, 3.0, 6.0, 1, nonoverlap=True)
+    assert sph.shape == (40, 4)
+    assert (sph[:, 3] >= 3.0).all() and (sph[:, 3] <= 6.0).all()
+    L = np.array(shape, float)
+    for i in range(40):
+        for j in range(i):
+            d = np.abs(sph[i, :3] - sph[j, :3])
+            d = np.minimum(d, L - d)
+            assert np.sqrt((d * d).sum()) >= sph[i, 3] + sph[j, 3]
+    assert np.array_equal(sph, random_spheres(shape, 40, 3.0, 6.0, 1, nonoverlap=True))
+    _, fluid, rho = config1_case()
+    # 40 disjoint balls of radius 3..6 occupy 3.4%..28% of the box
+    assert 0.70 < fluid.mean() < 0.97 and rho.dtype == np.float32
