@@ -9,10 +9,13 @@ one stream-collide launch over the whole lattice (by default the compact
 kernel, which stores and visits fluid nodes only).
 
 Prints one JSON line: value = fluid-node MLUPS of the FP32 form (whole job,
-inputs resident in HBM), the roofline object of the fused kernel (algorithmic
-bytes 2*19*4 = 152 per fluid update / HIP-event-timed average launch
-duration on the context stream), the FP16-storage form beside it, and the
-CPU baseline (the oracle, OpenMP, on a bounded sample of the same workload).
+inputs resident in HBM), the roofline object of the step kernel (the compact
+stream-collide kernel unless --kernel says otherwise; algorithmic bytes
+2*19*4 = 152 per fluid update / HIP-event-timed average launch duration on
+the context stream; traffic from the rocprofv3 counter runs recorded in
+profiles/traffic_512.txt), the FP16- and BF16-storage forms beside it with
+their error against the FP64 path, and the CPU baseline (the oracle,
+OpenMP, on a bounded sample of the same workload).
 """
 from __future__ import annotations
 
