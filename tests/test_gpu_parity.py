@@ -327,3 +327,28 @@ def test_pressure_faces_on_y(kernel):
     st, (g_rho, g_p, g_u, g_f, _) = _run_pair(shape, fluid, rho, 12, "fp32", kernel, **params)
     assert np.isfinite(st.fA[fluid]).all()
     assert np.array_equal(g_f[fluid], st.fA[fluid].astype(np.float64))
+
+
+@pytest.mark.parametrize("kernel", ["compact", "fused", "twopass"])
+@pytest.mark.parametrize("geometry", ["all_solid", "one_fluid_node", "all_fluid"])
+def test_degenerate_geometries(geometry, kernel):
+    """Edge cases of the geometry: no fluid node at all (work lists and
+    rank ranges are empty, every step is a no-op), a single fluid node
+    enclosed by solids (every direction bounces back), and a fully fluid
+    periodic box (no bounce-back, every rank range full: 17 x 9 x 40 spans
+    two z blocks and ragged tiles).  Bit-identical to the oracle where there
+    are fluid nodes; mass exactly zero without them."""
+    shape = (7, 9, 40) if geometry != "all_fluid" else (17, 9, 40)
+    fluid = np.zeros(shape, bool) if geometry != "all_fluid" else np.ones(shape, bool)
+    if geometry == "one_fluid_node":
+        fluid[3, 4, 33] = True
+    rho = noisy_rho(shape, 0.7, 0.01, 11)
+    print(f"case degenerate {geometry} {kernel} shape={shape}", flush=True)
+    st, (g_rho, g_p, g_u, g_f, stats) = _run_pair(shape, fluid, rho, 12, "fp32", kernel, **SC)
+    assert stats["nonfinite"] == 0
+    if not fluid.any():
+        assert stats["mass"] == 0.0
+        return
+    assert np.array_equal(g_f[fluid], st.fA[fluid].astype(np.float64))
+    assert np.array_equal(g_rho[fluid], st.rho[fluid].astype(np.float64))
+    assert abs(stats["mass"] / float(st.fA[fluid].astype(np.float64).sum()) - 1) < 1e-6
