@@ -79,10 +79,11 @@ struct Eos {
   int carn_star;
 };
 
-// dPress (include/iteration_device.cuh:7-27)
-template <class R>
+// dPress (include/iteration_device.cuh:7-27).  kCS = false compiles the
+// Shan-Chen form only (for callers that know carn_star is 0).
+template <bool kCS = true, class R>
 __host__ __device__ inline R eos_pressure(const Eos<R>& e, R rho) {
-  if (e.carn_star) {
+  if (kCS && e.carn_star) {
     // a = R = 1 and b = 4 in the reference; products with 1 are exact
     const R eta = (R)((double)((R)4.0 * rho) / 4.0);
     const R eta2 = eta * eta;
@@ -97,10 +98,10 @@ __host__ __device__ inline R eos_pressure(const Eos<R>& e, R rho) {
   return e.cs2 * rho + e.half_cs2G * (psi * psi);
 }
 
-// dPsi (include/iteration_device.cuh:31-38)
-template <class R>
+// dPsi (include/iteration_device.cuh:31-38); kCS as in eos_pressure.
+template <bool kCS = true, class R>
 __host__ __device__ inline R eos_psi(const Eos<R>& e, R rho) {
-  if (e.carn_star) {
+  if (kCS && e.carn_star) {
     const R p = eos_pressure(e, rho);
     return (R)__builtin_sqrt(2.0 * (double)(p - e.cs2 * rho) / (double)(e.cs2 * e.G));
   }
