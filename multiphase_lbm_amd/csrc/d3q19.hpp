@@ -38,6 +38,19 @@ __host__ __device__ constexpr float inv_weight_f(int i) {
   return wclass(i) == 0 ? 3.0f : (wclass(i) == 1 ? 18.0f : 36.0f);
 }
 
+// A double constant as a wave-uniform scalar operand at its point of use.
+// In device code the empty asm makes the value opaque where it is used, so
+// the compiler materialises it into a scalar register pair there instead of
+// keeping a vector-register copy live across the caller's loop (the 13
+// polynomial coefficients below held ~20 VGPRs of the marching kernel).
+// The value is unchanged.
+__host__ __device__ inline double sconst(double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+s"(c));
+#endif
+  return c;
+}
+
 // exp in double: Cody-Waite reduction with the fdlibm ln2 split and a
 // degree-13 Taylor polynomial evaluated by Horner's rule with fused
 // multiply-adds.  Every operation is an IEEE-754 double operation, so the
@@ -51,17 +64,17 @@ __host__ __device__ inline double exp_cw(double x) {
   const double kLn2Lo = 1.90821492927058770002e-10;
   const double k = __builtin_rint(x * kLog2e);
   const double r = (x - k * kLn2Hi) - k * kLn2Lo;
-  double p = 1.0 / 6227020800.0;
-  p = __builtin_fma(p, r, 1.0 / 479001600.0);
-  p = __builtin_fma(p, r, 1.0 / 39916800.0);
-  p = __builtin_fma(p, r, 1.0 / 3628800.0);
-  p = __builtin_fma(p, r, 1.0 / 362880.0);
-  p = __builtin_fma(p, r, 1.0 / 40320.0);
-  p = __builtin_fma(p, r, 1.0 / 5040.0);
-  p = __builtin_fma(p, r, 1.0 / 720.0);
-  p = __builtin_fma(p, r, 1.0 / 120.0);
-  p = __builtin_fma(p, r, 1.0 / 24.0);
-  p = __builtin_fma(p, r, 1.0 / 6.0);
+  double p = sconst(1.0 / 6227020800.0);
+  p = __builtin_fma(p, r, sconst(1.0 / 479001600.0));
+  p = __builtin_fma(p, r, sconst(1.0 / 39916800.0));
+  p = __builtin_fma(p, r, sconst(1.0 / 3628800.0));
+  p = __builtin_fma(p, r, sconst(1.0 / 362880.0));
+  p = __builtin_fma(p, r, sconst(1.0 / 40320.0));
+  p = __builtin_fma(p, r, sconst(1.0 / 5040.0));
+  p = __builtin_fma(p, r, sconst(1.0 / 720.0));
+  p = __builtin_fma(p, r, sconst(1.0 / 120.0));
+  p = __builtin_fma(p, r, sconst(1.0 / 24.0));
+  p = __builtin_fma(p, r, sconst(1.0 / 6.0));
   p = __builtin_fma(p, r, 0.5);
   p = __builtin_fma(p, r, 1.0);
   p = __builtin_fma(p, r, 1.0);

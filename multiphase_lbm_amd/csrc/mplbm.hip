@@ -331,7 +331,7 @@ __device__ __forceinline__ void collide(typename Mo::C* f, typename Mo::C rho, t
       }
       const C trho = weight<C>(i) * rho;
       const double deu = (double)eu;
-      const double poly = 1.0 + 3.0 * deu + 4.5 * deu * deu - usq15;
+      const double poly = 1.0 + sconst(3.0) * deu + sconst(4.5) * deu * deu - usq15;
       const double fi = (double)f[i];
       f[i] = (C)(fi + ((double)trho * poly - fi) * itau);
     }
@@ -1395,17 +1395,21 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
   }
 }
 
-// Occupancy the node-shifted 16-bit kernels without faces are held to: 6
-// waves per SIMD (80 VGPRs), three 512-thread workgroups per CU like the
-// global-shift kernel (they would take 81 and drop to two); 1 leaves the
-// register budget to the compiler (the face variants would spill at 80).
-template <class Mo, bool kNS, bool kFaces>
+// Occupancy the kernels without faces are held to where they fit without
+// spilling: 6 waves per SIMD (80 VGPRs), three 512-thread workgroups per CU
+// instead of two, for the node-shifted 16-bit kernels (81 left alone) and
+// the plain FP32 kernels (83 and 82 left alone; 512^3 FP32 FAST 2.72 ->
+// 2.52 ms); 1 leaves the register budget to the compiler (the other
+// variants would spill at 80).
+template <class Mo, bool kNS, bool kFaces, bool kPlain>
 constexpr int compact_min_waves() {
-  return kNS && !kFaces ? 6 : 1;
+  if constexpr (kFaces) return 1;
+  return (kNS || (kPlain && Mo::kStorage == MPLBM_FP32)) ? 6 : 1;
 }
 
 template <class Mo, bool kFaces, int kThreads, bool kSR = false, bool kNS = false, bool kPlain = false>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(compact_min_waves<Mo, kNS, kFaces>())))
+__global__ __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(compact_min_waves<Mo, kNS, kFaces, kPlain>())))
 void k_compact(Lattice<Mo> L, Phys<typename Mo::C> ph, CTiling T, CLists W) {
   extern __shared__ unsigned char smem_raw[];
   int b = blockIdx.x;
