@@ -3171,12 +3171,14 @@ struct EnsSampleC {
 // compiler takes 88 (5 waves), and the 12 bytes per thread it spills at 80
 // cost less than the wave gains (64 packings: FP16S 1.64 -> 1.53 ms per
 // step; FP32 FAST held to 96 was slower, 1.695 -> 1.709 ms).
-template <class Mo>
+template <class Mo, bool kPlain>
 constexpr int ensemble_min_waves() {
-  return sizeof(typename Mo::S) == 2 ? 6 : 1;
+  return (sizeof(typename Mo::S) == 2 || (kPlain && Mo::kStorage == MPLBM_FP32)) ? 6 : 1;
 }
-template <class Mo, int kBlock>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ensemble_min_waves<Mo>())))
+// kPlain as in compact_march: launches without macro writes when no sample
+// injects or uses the Carnahan-Starling EOS
+template <class Mo, int kBlock, bool kPlain = false>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ensemble_min_waves<Mo, kPlain>())))
 void k_ensemble_c(const EnsItem* __restrict__ items, int nitems,
                                                        const EnsSampleC<Mo>* __restrict__ samples, int parity,
                                                        int macro, EnsQueue* queue) {
@@ -3191,8 +3193,8 @@ void k_ensemble_c(const EnsItem* __restrict__ items, int nitems,
     Phys<typename Mo::C> ph = sd.ph;
     ph.write_macro = macro;
     const int tile = __builtin_amdgcn_readfirstlane(it.tyi * sd.T.tiles_z + it.tzi);
-    compact_march<Mo, false, kBlock>(sd.L[parity], ph, sd.T, sd.W, __builtin_amdgcn_readfirstlane(it.seg), tile,
-                                     smem_raw);
+    compact_march<Mo, false, kBlock, false, false, kPlain>(sd.L[parity], ph, sd.T, sd.W,
+                                                           __builtin_amdgcn_readfirstlane(it.seg), tile, smem_raw);
     __syncthreads();  // the next item reuses the LDS regions
   }
   ens_finish(queue);
@@ -3395,15 +3397,26 @@ static mplbm_status ens_launch(mplbm_ensemble* e, int parity, int macro) {
     const int grid = std::max(1, std::min(nitems, ncu * per_cu));
     if (e->compact) {
       const auto* sc = static_cast<const EnsSampleC<Mo>*>(e->d_samples);
-      if (e->block == 512)
-        hipLaunchKernelGGL((k_ensemble_c<Mo, 512>), dim3(grid), dim3(512), e->lds_bytes, e->stream, e->d_items,
-                           nitems, sc, parity, macro, e->d_queue);
-      else if (e->block == 256)
-        hipLaunchKernelGGL((k_ensemble_c<Mo, 256>), dim3(grid), dim3(256), e->lds_bytes, e->stream, e->d_items,
-                           nitems, sc, parity, macro, e->d_queue);
-      else if constexpr (Mo::kStorage != MPLBM_FP64)
-        hipLaunchKernelGGL((k_ensemble_c<Mo, 1024>), dim3(grid), dim3(1024), e->lds_bytes, e->stream, e->d_items,
-                           nitems, sc, parity, macro, e->d_queue);
+      // the plain kernel when this launch writes no macro fields and no
+      // sample injects (a zero increment adds nothing) or uses the
+      // Carnahan-Starling EOS
+      bool plain = !macro;
+      for (const mplbm_ctx* c : e->ctx)
+        plain = plain && !c->cfg.carn_star && !(c->cfg.rho_inject_period > 0 && c->cfg.rho_increment != 0.0);
+      auto go = [&](auto kernel, int block) {
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), e->lds_bytes, e->stream, e->d_items, nitems, sc, parity,
+                           macro, e->d_queue);
+      };
+      if (e->block == 512) {
+        if (plain) go(k_ensemble_c<Mo, 512, true>, 512);
+        else go(k_ensemble_c<Mo, 512>, 512);
+      } else if (e->block == 256) {
+        if (plain) go(k_ensemble_c<Mo, 256, true>, 256);
+        else go(k_ensemble_c<Mo, 256>, 256);
+      } else if constexpr (Mo::kStorage != MPLBM_FP64) {
+        if (plain) go(k_ensemble_c<Mo, 1024, true>, 1024);
+        else go(k_ensemble_c<Mo, 1024>, 1024);
+      }
       ENS_HIP(e, hipGetLastError());
       return MPLBM_OK;
     }
