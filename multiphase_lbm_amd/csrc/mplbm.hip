@@ -1212,7 +1212,7 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
     }
     hasN = false;
     if (tid < npull) {
-      if constexpr (sizeof(S) == 4) pN = pe;  // entry loaded one plane ahead
+      if constexpr (sizeof(S) == 4 || (sizeof(S) == 2 && kPlain)) pN = pe;  // entry loaded one plane ahead
       else pN = W.pos[first + tid];
       int gy, gz;
       S raw[kQ];
@@ -1324,17 +1324,18 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
     const PlaneWork w = plane_work(x0);
     psi_plane(x0, true, w, entry_of(w));
   }
-  // With 4-byte storage the list entry of the next plane's pull is
-  // loaded one plane ahead, before this plane's population loads, and kept
-  // in this thread's LDS word (written after the pull has waited for its
-  // loads), so no wait on it precedes the pull's address computation: such
-  // a wait (waiting for the entry alone, 512^3 FP32 3.18 -> 3.06 ms) also
-  // waits for the stores of the previous collision, as loads and stores
-  // share one counter.  The 16-bit kernels load the entry in the pull: they
+  // With 4-byte storage, and in the plain 16-bit kernels, the list entry of
+  // the next plane's pull is loaded one plane ahead, before this plane's
+  // population loads, and kept in this thread's LDS word (written after the
+  // pull has waited for its loads), so no wait on it precedes the pull's
+  // address computation: such a wait (waiting for the entry alone, 512^3
+  // FP32 3.18 -> 3.06 ms, plain FP16S 2.05 -> 1.95 ms) also waits for the
+  // stores of the previous collision, as loads and stores share one
+  // counter.  The general 16-bit kernels load the entry in the pull: they
   // run at 80 registers, and one more held across the pull splits its 19
-  // loads into two batches (FP16S 2.11 -> 2.18 ms); FP64 would drop from
-  // three waves per SIMD to two.
-  constexpr bool kAhead = sizeof(S) == 4;  // as in psi_plane
+  // loads into two batches (FP16S 2.11 -> 2.18 ms); the plain ones have the
+  // room (72 -> 73 VGPRs).  FP64 would drop from three waves per SIMD to two.
+  constexpr bool kAhead = sizeof(S) == 4 || (sizeof(S) == 2 && kPlain);  // as in psi_plane
   PlaneWork wc = plane_work(x0 + 1);
   if constexpr (kAhead) pent[tid] = entry_of(wc);
 
