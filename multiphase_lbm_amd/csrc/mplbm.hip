@@ -1030,18 +1030,28 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
       if (pi >= 0) psl[slot(xp) * T.PYZ + pi] = psi_solid;
     }
   };
+  // the thread's rank-region positions are recomputed from an opaque copy
+  // of its index at each use instead of being kept as LDS addresses across
+  // the march (a register the plain FP32 kernel would otherwise spill)
+  auto thread_index = [&]() {
+    int t = tid;
+    asm volatile("" : "+v"(t));
+    return t;
+  };
   auto expand_issue = [&](int xp, uint2* w) {
     const uint2* rwp = L.rw + (uint64_t)wrapx(xp) * L.ly * L.W * kRankGroups;
+    const int t = thread_index();
 #pragma unroll
     for (int m = 0; m < kPre; ++m) {
-      const int q = tid + m * kThreads;
+      const int q = t + m * kThreads;
       if (q < T.RYZ) w[m] = rw_word(rwp, q);
     }
   };
   auto expand_finish = [&](int xp, const uint2* w) {
+    const int t = thread_index();
 #pragma unroll
     for (int m = 0; m < kPre; ++m) {
-      const int q = tid + m * kThreads;
+      const int q = t + m * kThreads;
       if (q < T.RYZ) put_rank(xp, q, w[m]);
     }
     const uint2* rwp = L.rw + (uint64_t)wrapx(xp) * L.ly * L.W * kRankGroups;
