@@ -961,7 +961,8 @@ struct CLists {
 // kPlain: a step with no injection, no macro-field write and the Shan-Chen
 // EOS, compiled without those paths (fewer registers: FP32 109 -> 101
 // VGPRs, 16-bit 80 -> 72; 2-3% faster).
-template <class Mo, bool kFaces, int kThreads, bool kSR = false, bool kNS = false, bool kPlain = false>
+// kPre: rank words of the rank region this thread loads ahead (below).
+template <class Mo, bool kFaces, int kThreads, bool kSR = false, bool kNS = false, bool kPlain = false, int kPre = 2>
 __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<typename Mo::C>& ph, const CTiling& T,
                                               const CLists& W, int sg, int tile, unsigned char* smem_raw) {
   using C = typename Mo::C;
@@ -1013,7 +1014,7 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
   // (1) rank region and solid psi of plane xp.  expand_issue loads the
   // rank words of the first kPre positions of this thread (so their latency
   // overlaps the pulls that follow), expand_finish writes the region.
-  constexpr int kPre = 2;
+  static_assert(kPre >= 1 && kPre <= 2, "one or two rank words ahead");
   auto rw_word = [&](const uint2* rwp, int q) { return rwp[qtab[q] >> 5]; };
   // the node shift of a fluid cell of the rank region goes to LDS with its
   // rank (kNS)
@@ -1055,7 +1056,7 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
       if (q < T.RYZ) put_rank(xp, q, w[m]);
     }
     const uint2* rwp = L.rw + (uint64_t)wrapx(xp) * L.ly * L.W * kRankGroups;
-    for (int q = tid + kPre * kThreads; q < T.RYZ; q += kThreads) put_rank(xp, q, rw_word(rwp, q));
+    for (int q = thread_index() + kPre * kThreads; q < T.RYZ; q += kThreads) put_rank(xp, q, rw_word(rwp, q));
   };
   auto expand = [&](int xp) {
     uint2 w[kPre];
@@ -1410,11 +1411,20 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
 // spilling: 6 waves per SIMD (80 VGPRs), three 512-thread workgroups per CU
 // instead of two, for the node-shifted 16-bit kernels (81 left alone) and
 // the plain FP32 kernels (83 and 82 left alone; 512^3 FP32 FAST 2.72 ->
-// 2.52 ms); 1 leaves the register budget to the compiler (the other
+// 2.52 ms); 8 waves (64 VGPRs, four 512-thread workgroups per CU, whose
+// 40,000 B of LDS each fill the CU's 160 KiB) for the plain 16-bit kernels
+// without node shifts (73 left alone; FP16S 1.94 -> 1.93 ms, stochastic
+// rounding 2.07 -> 2.04 ms; they load one rank word ahead, as two would
+// spill 4 VGPRs); 1 leaves the register budget to the compiler (the other
 // variants would spill at 80).
+template <class Mo, bool kNS, bool kFaces, bool kPlain>
+constexpr bool compact_lean() {
+  return !kFaces && kPlain && !kNS && sizeof(typename Mo::S) == 2;
+}
 template <class Mo, bool kNS, bool kFaces, bool kPlain>
 constexpr int compact_min_waves() {
   if constexpr (kFaces) return 1;
+  if constexpr (compact_lean<Mo, kNS, kFaces, kPlain>()) return 8;
   return (kNS || (kPlain && Mo::kStorage == MPLBM_FP32)) ? 6 : 1;
 }
 
@@ -1431,8 +1441,9 @@ void k_compact(Lattice<Mo> L, Phys<typename Mo::C> ph, CTiling T, CLists W) {
   if (b >= T.nblocks) return;  // whole padding block leaves before any barrier
   const int ntile = T.tiles_y * T.tiles_z;
   const int k = b / ntile;
-  compact_march<Mo, kFaces, kThreads, kSR, kNS, kPlain>(L, ph, T, W, T.seg0 + k * T.seg_stride, b - k * ntile,
-                                                        smem_raw);
+  constexpr int kPre = compact_lean<Mo, kNS, kFaces, kPlain>() ? 1 : 2;
+  compact_march<Mo, kFaces, kThreads, kSR, kNS, kPlain, kPre>(L, ph, T, W, T.seg0 + k * T.seg_stride, b - k * ntile,
+                                                              smem_raw);
 }
 
 // Store population i of the fluid node of rank r in plane x of a compact
