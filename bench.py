@@ -241,11 +241,17 @@ def error_check(a, shape, fluid, rho, storages):
     return out
 
 
-def traffic_from_profiles(storage, tag):
+TRAFFIC_WORKLOAD = (512, 0.45, 6.0, 18.0, 3)  # size, porosity, rmin, rmax, seed of profiles/traffic_512.txt
+
+
+def traffic_from_profiles(a, storage, tag):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE +
     WRITE_SIZE, each scaled by the copy-kernel calibration) for this
     workload and kernel configuration, from the `traffic` lines of
-    profiles/traffic_512.txt; None when not measured."""
+    profiles/traffic_512.txt; None when not measured (another packing or
+    size, or a configuration without a line)."""
+    if (a.size, a.porosity, a.rmin, a.rmax, a.seed) != TRAFFIC_WORKLOAD:
+        return None
     path = os.path.join(ROOT, "profiles", "traffic_512.txt")
     try:
         with open(path) as f:
@@ -689,7 +695,7 @@ def main():
                    "mlups_unit": "fluid-node updates per second / 1e6"},
         "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles("fp32", r["tag"]),
+                     "traffic": traffic_from_profiles(a, "fp32", r["tag"]),
                      "bytes_per_update": r["bytes_per_update"], "kernel_ms": round(r["kernel_ms"], 4)},
         "cpu_baseline": cpu,
         "error": errors.get("fp32"),
@@ -702,7 +708,7 @@ def main():
                    "error": errors.get(s),
                    "roofline": {"bound": "hbm", "achieved": round(q["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                                 "unit": "GB/s", "frac": round(q["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                                "traffic": traffic_from_profiles(s, q["tag"]), "bytes_per_update": q["bytes_per_update"],
+                                "traffic": traffic_from_profiles(a, s, q["tag"]), "bytes_per_update": q["bytes_per_update"],
                                 "kernel_ms": round(q["kernel_ms"], 4)}}
     print(json.dumps(line), flush=True)
 
