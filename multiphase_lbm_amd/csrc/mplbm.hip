@@ -1993,9 +1993,14 @@ static mplbm_status choose_ctiling(mplbm_ctx* ctx) {
       return fail(ctx, MPLBM_ERR_ARG, "tile %dx%d needs %zu bytes of LDS (max %zu)", ty, tz, lds, kMaxGroupLds);
   }
   const int64_t tiles = (int64_t)T.tiles_y * T.tiles_z;
+  // segments of 48 planes where they leave at least 2048 workgroups, else
+  // 32, 16, 8 or 4: fewer set-up planes per marched plane against a longer
+  // tail of the last workgroups (512^3 FP32 2.63 / 2.61 / 2.69 ms and FP16S
+  // 1.93 / 1.90 / 1.96 ms at 32 / 48 / 64 planes, same device; 1024^3 FP32
+  // 2-4% and FP16S 3-7% faster at 48 than at 32)
   if (seg <= 0) {
-    seg = 32;
-    while (seg > 4 && tiles * ((c.lx + seg - 1) / seg) < 2048) seg /= 2;
+    seg = 48;
+    while (seg > 4 && tiles * ((c.lx + seg - 1) / seg) < 2048) seg = seg == 48 ? 32 : seg / 2;
   }
   if (seg > c.lx) seg = (int)c.lx;
   T.seg = seg;
