@@ -36,7 +36,7 @@ def compact_candidates(lx, ly, lz):
     for tz, ty in itertools.product(tzs, [4, 8, 12, 16, 24, 32]):
         if ty > ly or ty * tz > 2048:
             continue
-        for seg in (16, 32, 64):
+        for seg in (16, 32, 48, 64):
             if seg <= lx:
                 out.append((ty, tz, seg))
     return out

@@ -187,7 +187,7 @@ def tile_candidates(block):
     return [(ty, tz) for ty, tz in tiles if block // 2 <= ty * tz <= block]
 
 
-def autotune(make_ensemble, shapes, blocks=(256, 512), segs=(16, 32, 64), steps=4, log=None):
+def autotune(make_ensemble, shapes, blocks=(256, 512), segs=(16, 32, 48, 64), steps=4, log=None):
     """Ensemble autotuner: for each workgroup size, a coordinate search over
     the sample-size classes (largest first) that picks the compact tile
     (tile_y, tile_z) with 32-plane segments, then the segment length, of
