@@ -1997,9 +1997,14 @@ static mplbm_status choose_ctiling(mplbm_ctx* ctx) {
   // 32, 16, 8 or 4: fewer set-up planes per marched plane against a longer
   // tail of the last workgroups (512^3 FP32 2.63 / 2.61 / 2.69 ms and FP16S
   // 1.93 / 1.90 / 1.96 ms at 32 / 48 / 64 planes, same device; 1024^3 FP32
-  // 2-4% and FP16S 3-7% faster at 48 than at 32)
+  // 2-4% and FP16S 3-7% faster at 48 than at 32).  With pressure faces on x
+  // the first and last segments run the face variant in a launch of their
+  // own, which longer segments lengthen: 32 planes there (768x512x512
+  // drainage FP32 3.86 / 3.97 / 3.85 ms and FP16S 2.90 / 3.00 / 2.90 ms at
+  // 32 / 48 / 24 planes, same device)
   if (seg <= 0) {
-    seg = 48;
+    const bool x_faces = c.face_type[0] != 0 || c.face_type[1] != 0;
+    seg = x_faces ? 32 : 48;
     while (seg > 4 && tiles * ((c.lx + seg - 1) / seg) < 2048) seg = seg == 48 ? 32 : seg / 2;
   }
   if (seg > c.lx) seg = (int)c.lx;
