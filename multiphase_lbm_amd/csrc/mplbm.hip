@@ -934,6 +934,7 @@ struct CTiling {
   int xcd_remap;
   int threads;           // workgroup size (256, 512 or 1024)
   int seg0, seg_stride;  // a launch covers segments seg0 + k*seg_stride, k < nblocks / tiles
+  int xoff, xlim;        // segment sg spans planes xoff + sg*seg .. min(+seg, xlim) (xlim 0: lx)
 };
 
 struct CLists {
@@ -979,8 +980,8 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
   const int tyi = tile / T.tiles_z;
   const int tzi = tile - tyi * T.tiles_z;
   const int y0 = tyi * T.TYi, z0 = tzi * T.TZi;
-  const int x0 = __builtin_amdgcn_readfirstlane(sg * T.seg);
-  const int x1 = __builtin_amdgcn_readfirstlane(min(x0 + T.seg, L.lx));
+  const int x0 = __builtin_amdgcn_readfirstlane(T.xoff + sg * T.seg);
+  const int x1 = __builtin_amdgcn_readfirstlane(min(x0 + T.seg, T.xlim > 0 ? T.xlim : L.lx));
   const int n = x1 - x0;
   const int tid = threadIdx.x;
 
@@ -1997,14 +1998,9 @@ static mplbm_status choose_ctiling(mplbm_ctx* ctx) {
   // 32, 16, 8 or 4: fewer set-up planes per marched plane against a longer
   // tail of the last workgroups (512^3 FP32 2.63 / 2.61 / 2.69 ms and FP16S
   // 1.93 / 1.90 / 1.96 ms at 32 / 48 / 64 planes, same device; 1024^3 FP32
-  // 2-4% and FP16S 3-7% faster at 48 than at 32).  With pressure faces on x
-  // the first and last segments run the face variant in a launch of their
-  // own, which longer segments lengthen: 32 planes there (768x512x512
-  // drainage FP32 3.86 / 3.97 / 3.85 ms and FP16S 2.90 / 3.00 / 2.90 ms at
-  // 32 / 48 / 24 planes, same device)
+  // 2-4% and FP16S 3-7% faster at 48 than at 32)
   if (seg <= 0) {
-    const bool x_faces = c.face_type[0] != 0 || c.face_type[1] != 0;
-    seg = x_faces ? 32 : 48;
+    seg = 48;
     while (seg > 4 && tiles * ((c.lx + seg - 1) / seg) < 2048) seg = seg == 48 ? 32 : seg / 2;
   }
   if (seg > c.lx) seg = (int)c.lx;
@@ -2267,21 +2263,42 @@ static mplbm_status launch_step(mplbm_ctx* ctx, int64_t step, int32_t reverse, u
       launch_as(std::false_type{}, std::false_type{}, R, faces);
     };
     const bool x_faces_only = ph.any_face && !(ph.face[2] | ph.face[3] | ph.face[4] | ph.face[5]);
-    // a middle segment [x0, x1) pulls planes x0-1 .. x1 for its psi ring, so
-    // the plain variant may run it only if neither ring plane is a face
-    // plane: x0-1 >= 1 needs seg >= 2, x1 <= lx-2 needs a last segment of at
-    // least two planes
-    const int last_len = (int)ctx->cfg.lx - (T.nseg - 1) * T.seg;
-    if (x_faces_only && T.nseg >= 3 && T.seg >= 2 && last_len >= 2) {
-      // face rules and the skipped force only touch planes 0 and lx-1: the
-      // first and last segments run the face variant, the others the plain
-      // one (fewer registers), in two launches
+    // Face rules and the skipped force only touch planes 0 and lx-1, so with
+    // faces on x alone the step runs in two launches: end segments [0, e)
+    // and [s*e, lx) in the face variant (segments 0 and s of length e), the
+    // planes between in segments of the tiling's length in the plain
+    // variant (fewer registers).  A middle segment [x0, x1) pulls planes
+    // x0-1 .. x1 for its psi ring, which are no face planes when e >= 2 and
+    // the last end holds at least two planes.  e is the first of 8 .. 2,
+    // 9 .. 16 with s = ceil((lx-e)/e) leaving the last end 2 .. e planes and
+    // the middle at least two; short ends keep the slower face variant to at
+    // most 2e planes and its launch short (768x512x512 drainage FP32 3.85 ->
+    // 3.71 ms, FP16S 2.90 -> 2.75 ms against end segments of the tiling's
+    // 32 planes, same device).
+    const int lx = (int)ctx->cfg.lx;
+    int e = 0, s_end = 0;
+    for (const int c : {8, 7, 6, 5, 4, 3, 2, 9, 10, 11, 12, 13, 14, 15, 16}) {
+      if (!x_faces_only) break;
+      const int s = (lx - 1) / c;  // ceil((lx - c) / c)
+      if (lx - s * c >= 2 && (s - 1) * c >= 2) {
+        e = c;
+        s_end = s;
+        break;
+      }
+    }
+    if (e > 0) {
       const int ntile = T.tiles_y * T.tiles_z;
       CTiling mid = T, ends = T;
-      mid.seg0 = 1;
-      mid.nblocks = ntile * (T.nseg - 2);
+      mid.xoff = e;
+      mid.xlim = s_end * e;
+      mid.seg0 = 0;
+      mid.seg_stride = 1;
+      mid.nblocks = ntile * ((mid.xlim - e + T.seg - 1) / T.seg);
+      ends.seg = e;
+      ends.xoff = 0;
+      ends.xlim = lx;
       ends.seg0 = 0;
-      ends.seg_stride = T.nseg - 1;
+      ends.seg_stride = s_end;
       ends.nblocks = ntile * 2;
       launch(mid, false);
       launch(ends, true);

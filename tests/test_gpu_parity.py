@@ -218,9 +218,9 @@ def test_carnahan_starling(kernel):
 @pytest.mark.parametrize("kernel,seg", [("fused", 0), ("twopass", 0), ("compact", 0), ("compact", 16),
                                         ("compact", 7)])
 def test_pressure_faces_and_body_force(kernel, seg):
-    """x faces only: the compact kernel runs its face variant on the first
-    and last segments and the plain variant elsewhere (seg 0: 4-plane
-    segments; 16: one segment; 7: a short last segment)."""
+    """x faces only: the compact kernel runs its face variant on end
+    segments [0, 7) and [14, 16) and the plain variant on the planes between
+    (seg 0: the default length; 16: one middle segment; 7: one of 7)."""
     shape = (16, 12, 10)
     fluid, rho = _case(shape, 2, 2.0, 3.0, seed=9, rho0=1.0, amp=0.05)
     print(f"case pressure faces {kernel} seg {seg}", flush=True)
@@ -234,12 +234,14 @@ def test_pressure_faces_and_body_force(kernel, seg):
     assert np.array_equal(g_u[fluid], st.u[fluid].astype(np.float64))
 
 
-@pytest.mark.parametrize("lx,seg", [(15, 7), (29, 7), (16, 7), (17, 8)])
+@pytest.mark.parametrize("lx,seg", [(15, 7), (29, 7), (16, 7), (17, 8), (7, 3), (841, 48)])
 def test_pressure_faces_short_last_segment(lx, seg):
-    """x faces with a last segment of one plane (lx = k*seg + 1: the plane
-    before it is the outlet face, so no segment may run the plain variant
-    over it) and of two planes (the split into plain and face launches
-    stays on)."""
+    """x faces on lengths that exercise the launch plans: end segments in the
+    face variant of 6 and 3 planes (lx 15), 8 and 5 (29), 7 and 2 (16), 7 and
+    3 (17), 9 and 4 (841, where no end of 2 .. 8 planes leaves a last end of
+    two or more) with the planes between in middle segments of seg planes,
+    a short last one included; lx 7 has no such plan and runs every segment
+    in the face variant."""
     shape = (lx, 12, 10)
     fluid, rho = _case(shape, 2, 2.0, 3.0, seed=lx, rho0=1.0, amp=0.05)
     print(f"case pressure faces lx {lx} seg {seg}", flush=True)
