@@ -155,6 +155,10 @@ __device__ __forceinline__ typename Mo::S encode(typename Mo::C v, int i, const 
 
 // Node-shifted form: w_i (g / scale + shift) and (f / w_i - shift) * scale,
 // shift the 16-bit stored density of the node that wrote the population.
+// kSolidShift marks a solid cell in the LDS shift region: a NaN bit pattern,
+// so only a node whose stored density is already this NaN would be decoded
+// with the pulling node's own shift instead of its own.
+constexpr uint16_t kSolidShift = 0xFFFFu;
 template <class Mo>
 __device__ __forceinline__ float shift_value(typename Mo::S s) {
   if constexpr (Mo::kStorage == MPLBM_FP16S) return __half2float(s);
@@ -1018,14 +1022,15 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
   static_assert(kPre >= 1 && kPre <= 2, "one or two rank words ahead");
   auto rw_word = [&](const uint2* rwp, int q) { return rwp[qtab[q] >> 5]; };
   // the node shift of a fluid cell of the rank region goes to LDS with its
-  // rank (kNS)
+  // rank, kSolidShift for a solid cell, so a decode tells a bounce-back from
+  // the shift alone (kNS)
   auto put_rank = [&](int xp, int q, uint2 w) {
     const uint32_t bit = qtab[q] & 31u;
     const bool fl = (w.x >> bit) & 1u;
     const uint32_t rank = w.y + __popc(w.x & ((1u << bit) - 1u));
     rm[slot(xp) * T.RYZ + q] = fl ? rank * sz : ~0u;
     if constexpr (kNS) {
-      if (fl) shr[slot(xp) * T.RYZ + q] = L.sin[sidx(L, wrapx(xp), rank)];
+      shr[slot(xp) * T.RYZ + q] = fl ? L.sin[sidx(L, wrapx(xp), rank)] : __builtin_bit_cast(S, kSolidShift);
     }
     if (!fl) {
       const int pi = qpsi[q];
@@ -1099,19 +1104,18 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
   // that wrote it (the source, or the node itself for a bounce-back), from
   // the shift regions of planes xp-1, xp, xp+1.  Returns the node's own shift.
   auto decode_shifted = [&](int xp, int q0, const S* raw, C* f) -> float {
-    const uint32_t* m0 = rm + slot(xp) * T.RYZ + q0;
-    const uint32_t* mm = rm + slot(xp - 1) * T.RYZ + q0;
-    const uint32_t* mq = rm + slot(xp + 1) * T.RYZ + q0;
     const S* s0 = shr + slot(xp) * T.RYZ + q0;
     const S* sm = shr + slot(xp - 1) * T.RYZ + q0;
     const S* sq = shr + slot(xp + 1) * T.RYZ + q0;
     const float own = shift_value<Mo>(s0[0]);
 #pragma unroll
     for (int i = 0; i < kQ; ++i) {
-      const uint32_t* mi = ex(i) == 1 ? mm : (ex(i) == -1 ? mq : m0);
       const S* si = ex(i) == 1 ? sm : (ex(i) == -1 ? sq : s0);
-      const int d = -ey(i) * T.RZ - ez(i);
-      const float sh = mi[d] != ~0u ? shift_value<Mo>(si[d]) : own;
+      const S v = si[-ey(i) * T.RZ - ez(i)];
+      float sh = own;
+      if constexpr (sizeof(S) == 2) {
+        if (__builtin_bit_cast(uint16_t, v) != kSolidShift) sh = shift_value<Mo>(v);
+      }
       f[i] = decode_at<Mo>(raw[i], i, ph, sh);
     }
     return own;
