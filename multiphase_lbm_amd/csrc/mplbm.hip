@@ -967,7 +967,11 @@ struct CLists {
 // EOS, compiled without those paths (fewer registers: FP32 109 -> 101
 // VGPRs, 16-bit 80 -> 72; 2-3% faster).
 // kPre: rank words of the rank region this thread loads ahead (below).
-template <class Mo, bool kFaces, int kThreads, bool kSR = false, bool kNS = false, bool kPlain = false, int kPre = 2>
+// kTZ: the tile's interior width in z when fixed at compile time (0: T.TZi),
+// so the row lengths of the rank and psi regions are constants and the
+// in-plane LDS reads of a pull or a force pair up into ds_read2 instructions.
+template <class Mo, bool kFaces, int kThreads, bool kSR = false, bool kNS = false, bool kPlain = false, int kPre = 2,
+          int kTZ = 0>
 __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<typename Mo::C>& ph, const CTiling& T,
                                               const CLists& W, int sg, int tile, unsigned char* smem_raw) {
   using C = typename Mo::C;
@@ -981,6 +985,8 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
   int* pent = qpsi + T.RYZ;                                                  // kThreads: next list entry
   typename Mo::S* shr = reinterpret_cast<typename Mo::S*>(pent + kThreads);  // kSlots x RYZ node shifts (kNS)
 
+  const int RZ = kTZ > 0 ? kTZ + 4 : T.RZ;  // row length of the rank region
+  const int PZ = kTZ > 0 ? kTZ + 2 : T.PZ;  // row length of the psi region
   const int tyi = tile / T.tiles_z;
   const int tzi = tile - tyi * T.tiles_z;
   const int y0 = tyi * T.TYi, z0 = tzi * T.TZi;
@@ -1001,10 +1007,10 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
   // and its index in the psi region (the same for every plane)
   for (int q = tid; q < T.RYZ; q += kThreads) {
     const int r = (int)__umulhi((uint32_t)q, T.magic_rz);
-    const int c = q - r * T.RZ;
+    const int c = q - r * RZ;
     const uint32_t z = (uint32_t)zcrd[c];
     qtab[q] = (rank_word(L.W, (uint32_t)ycrd[r], z) << 5) | (z & 31u);
-    qpsi[q] = (r >= 1 && r <= T.TYi + 2 && c >= 1 && c <= T.TZi + 2) ? (r - 1) * T.PZ + (c - 1) : -1;
+    qpsi[q] = (r >= 1 && r <= T.TYi + 2 && c >= 1 && c <= T.TZi + 2) ? (r - 1) * PZ + (c - 1) : -1;
   }
   __syncthreads();
 
@@ -1111,7 +1117,7 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
 #pragma unroll
     for (int i = 0; i < kQ; ++i) {
       const S* si = ex(i) == 1 ? sm : (ex(i) == -1 ? sq : s0);
-      const S v = si[-ey(i) * T.RZ - ez(i)];
+      const S v = si[-ey(i) * RZ - ez(i)];
       float sh = own;
       if constexpr (sizeof(S) == 2) {
         if (__builtin_bit_cast(uint16_t, v) != kSolidShift) sh = shift_value<Mo>(v);
@@ -1129,8 +1135,8 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
   auto pull = [&](int xp, int p, S* raw, C* f, int& gy, int& gz, uint32_t& self) -> C {
     const int xw = wrapx(xp);
     const int r = (int)__umulhi((uint32_t)p, T.magic_pz);
-    const int c = p - r * T.PZ;
-    const int q0 = (r + 1) * T.RZ + (c + 1);
+    const int c = p - r * PZ;
+    const int q0 = (r + 1) * RZ + (c + 1);
     gy = ycrd[r + 1];
     gz = zcrd[c + 1];
     const uint32_t* m0 = rm + slot(xp) * T.RYZ + q0;
@@ -1150,7 +1156,7 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
 #pragma unroll
     for (int i = 0; i < kQ; ++i) {
       const uint32_t* mi = ex(i) == 1 ? mm : (ex(i) == -1 ? mq : m0);
-      const uint32_t sr = mi[-ey(i) * T.RZ - ez(i)];
+      const uint32_t sr = mi[-ey(i) * RZ - ez(i)];
       // source in plane xp - e_x(i): sub-plane (1 - e_x)*19 + i from memory
       // plane xw; a solid source: own node, sub-plane 19 + opp(i)
       const bool fl = sr != ~0u;
@@ -1269,7 +1275,7 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
 #pragma unroll
       for (int i = 1; i < kQ; ++i) {
         const C* pp = ex(i) == 1 ? pzq : (ex(i) == -1 ? pzm : pz0);
-        pn[i] = pp[p + ey(i) * T.PZ + ez(i)];
+        pn[i] = pp[p + ey(i) * PZ + ez(i)];
       }
       sc_force(pn, pz0[p], ph, F);
     }
@@ -1388,7 +1394,7 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
     // (4) collide this thread's fluid interior node of plane x
     if (has) {
       const int r = (int)__umulhi((uint32_t)p, T.magic_pz);
-      const int gy = ycrd[r + 1], gz = zcrd[p - r * T.PZ + 1];
+      const int gy = ycrd[r + 1], gz = zcrd[p - r * PZ + 1];
       C f[kQ];
       if constexpr (kPackN) {
         float osh = 0.f;
@@ -1396,7 +1402,7 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
           S raw[kQ];
 #pragma unroll
           for (int i = 0; i < kQ; ++i) raw[i] = __builtin_bit_cast(S, (uint16_t)(fk[i / 2] >> (16 * (i & 1))));
-          osh = decode_shifted(x, (r + 1) * T.RZ + (p - r * T.PZ + 1), raw, f);
+          osh = decode_shifted(x, (r + 1) * RZ + (p - r * PZ + 1), raw, f);
         } else {
 #pragma unroll
           for (int i = 0; i < kQ; ++i)
@@ -1433,7 +1439,7 @@ constexpr int compact_min_waves() {
   return (kNS || (kPlain && Mo::kStorage == MPLBM_FP32)) ? 6 : 1;
 }
 
-template <class Mo, bool kFaces, int kThreads, bool kSR = false, bool kNS = false, bool kPlain = false>
+template <class Mo, bool kFaces, int kThreads, bool kSR = false, bool kNS = false, bool kPlain = false, int kTZ = 0>
 __global__ __launch_bounds__(kThreads)
 __attribute__((amdgpu_waves_per_eu(compact_min_waves<Mo, kNS, kFaces, kPlain>())))
 void k_compact(Lattice<Mo> L, Phys<typename Mo::C> ph, CTiling T, CLists W) {
@@ -1447,8 +1453,8 @@ void k_compact(Lattice<Mo> L, Phys<typename Mo::C> ph, CTiling T, CLists W) {
   const int ntile = T.tiles_y * T.tiles_z;
   const int k = b / ntile;
   constexpr int kPre = compact_lean<Mo, kNS, kFaces, kPlain>() ? 1 : 2;
-  compact_march<Mo, kFaces, kThreads, kSR, kNS, kPlain, kPre>(L, ph, T, W, T.seg0 + k * T.seg_stride, b - k * ntile,
-                                                              smem_raw);
+  compact_march<Mo, kFaces, kThreads, kSR, kNS, kPlain, kPre, kTZ>(L, ph, T, W, T.seg0 + k * T.seg_stride,
+                                                                   b - k * ntile, smem_raw);
 }
 
 // Store population i of the fluid node of rank r in plane x of a compact
@@ -2237,15 +2243,23 @@ static mplbm_status launch_step(mplbm_ctx* ctx, int64_t step, int32_t reverse, u
       // the plain variant (no faces) for steps that neither inject nor
       // write macro fields, with the Shan-Chen EOS
       const bool plain = !faces && !ph.inject_now && !ph.write_macro && !ph.eos.carn_star;
+      // the plain kernels of 32-wide tiles (the default) with the width fixed
+      const bool z32 = plain && R.TZi == 32;
       if (R.threads == 256) {
         k = faces ? k_compact<Mo, true, 256, kSR, kNS>
-                  : (plain ? k_compact<Mo, false, 256, kSR, kNS, true> : k_compact<Mo, false, 256, kSR, kNS>);
+                  : (z32     ? k_compact<Mo, false, 256, kSR, kNS, true, 32>
+                     : plain ? k_compact<Mo, false, 256, kSR, kNS, true>
+                             : k_compact<Mo, false, 256, kSR, kNS>);
       } else if (R.threads == 512) {
         k = faces ? k_compact<Mo, true, 512, kSR, kNS>
-                  : (plain ? k_compact<Mo, false, 512, kSR, kNS, true> : k_compact<Mo, false, 512, kSR, kNS>);
+                  : (z32     ? k_compact<Mo, false, 512, kSR, kNS, true, 32>
+                     : plain ? k_compact<Mo, false, 512, kSR, kNS, true>
+                             : k_compact<Mo, false, 512, kSR, kNS>);
       } else if constexpr (Mo::kStorage != MPLBM_FP64) {
         k = faces ? k_compact<Mo, true, 1024, kSR, kNS>
-                  : (plain ? k_compact<Mo, false, 1024, kSR, kNS, true> : k_compact<Mo, false, 1024, kSR, kNS>);
+                  : (z32     ? k_compact<Mo, false, 1024, kSR, kNS, true, 32>
+                     : plain ? k_compact<Mo, false, 1024, kSR, kNS, true>
+                             : k_compact<Mo, false, 1024, kSR, kNS>);
       }
       if (!k) return;
       // dynamic LDS above 64 KiB (gfx950 gives a workgroup up to 160 KiB)
