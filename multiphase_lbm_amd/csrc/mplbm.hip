@@ -3243,8 +3243,9 @@ constexpr int ensemble_min_waves() {
   return (sizeof(typename Mo::S) == 2 || (kPlain && Mo::kStorage == MPLBM_FP32)) ? 6 : 1;
 }
 // kPlain as in compact_march: launches without macro writes when no sample
-// injects or uses the Carnahan-Starling EOS
-template <class Mo, int kBlock, bool kPlain = false>
+// injects or uses the Carnahan-Starling EOS; kTZ as in compact_march, when
+// every sample's tile is that wide
+template <class Mo, int kBlock, bool kPlain = false, int kTZ = 0>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ensemble_min_waves<Mo, kPlain>())))
 void k_ensemble_c(const EnsItem* __restrict__ items, int nitems,
                                                        const EnsSampleC<Mo>* __restrict__ samples, int parity,
@@ -3260,7 +3261,7 @@ void k_ensemble_c(const EnsItem* __restrict__ items, int nitems,
     Phys<typename Mo::C> ph = sd.ph;
     ph.write_macro = macro;
     const int tile = __builtin_amdgcn_readfirstlane(it.tyi * sd.T.tiles_z + it.tzi);
-    compact_march<Mo, false, kBlock, false, false, kPlain>(sd.L[parity], ph, sd.T, sd.W,
+    compact_march<Mo, false, kBlock, false, false, kPlain, 2, kTZ>(sd.L[parity], ph, sd.T, sd.W,
                                                            __builtin_amdgcn_readfirstlane(it.seg), tile, smem_raw);
     __syncthreads();  // the next item reuses the LDS regions
   }
@@ -3474,14 +3475,20 @@ static mplbm_status ens_launch(mplbm_ensemble* e, int parity, int macro) {
         hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), e->lds_bytes, e->stream, e->d_items, nitems, sc, parity,
                            macro, e->d_queue);
       };
+      // every sample's tile 32 nodes wide: the plain kernel with that width
+      bool z32 = plain;
+      for (const mplbm_ctx* c : e->ctx) z32 = z32 && c->ctiling.TZi == 32;
       if (e->block == 512) {
-        if (plain) go(k_ensemble_c<Mo, 512, true>, 512);
+        if (z32) go(k_ensemble_c<Mo, 512, true, 32>, 512);
+        else if (plain) go(k_ensemble_c<Mo, 512, true>, 512);
         else go(k_ensemble_c<Mo, 512>, 512);
       } else if (e->block == 256) {
-        if (plain) go(k_ensemble_c<Mo, 256, true>, 256);
+        if (z32) go(k_ensemble_c<Mo, 256, true, 32>, 256);
+        else if (plain) go(k_ensemble_c<Mo, 256, true>, 256);
         else go(k_ensemble_c<Mo, 256>, 256);
       } else if constexpr (Mo::kStorage != MPLBM_FP64) {
-        if (plain) go(k_ensemble_c<Mo, 1024, true>, 1024);
+        if (z32) go(k_ensemble_c<Mo, 1024, true, 32>, 1024);
+        else if (plain) go(k_ensemble_c<Mo, 1024, true>, 1024);
         else go(k_ensemble_c<Mo, 1024>, 1024);
       }
       ENS_HIP(e, hipGetLastError());
