@@ -1184,8 +1184,11 @@ __device__ __forceinline__ void compact_march(const Lattice<Mo>& L, const Phys<t
   // and are pulled first, while only the previous plane's set is live.
   // 16-bit storage keeps the stored values of the kept node, two per
   // register, and decodes them again (with the face rules) at the collision;
-  // wider storage keeps the populations themselves
-  constexpr bool kPackN = sizeof(S) == 2;
+  // wider storage keeps the populations themselves, and so do the plain
+  // node-shifted kernels, whose LDS (not their registers) limits them to
+  // three workgroups per CU: the second decode would read the 19 source
+  // shifts again
+  constexpr bool kPackN = sizeof(S) == 2 && !(kNS && kPlain && !kFaces);
   constexpr int kNW = kPackN ? (kQ + 1) / 2 : kQ;
   using NT = std::conditional_t<kPackN, uint32_t, C>;
   NT fN[kNW];
