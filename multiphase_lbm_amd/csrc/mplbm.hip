@@ -1979,11 +1979,20 @@ static mplbm_status choose_ctiling(mplbm_ctx* ctx) {
   const mplbm_config& c = ctx->cfg;
   CTiling T{};
   int tz = c.tile_z, ty = c.tile_y, seg = c.seg_x;
-  // measured on 512^3 packings: 16 x 32 interior nodes (512 threads) for
-  // every storage form (16-bit: 2.22 ms against 2.34 ms for 8 x 32)
-  const int ty_def = 16;
+  // measured on 512^3 packings: 16 x 32 interior nodes (512 threads), and
+  // 32 x 32 (1024 threads, two workgroups per CU) for 16-bit storage with
+  // round-to-nearest and the global shift (FP16S 1.86 -> 1.83 ms, BF16S
+  // 1.86 -> 1.84 ms, same device; with stochastic rounding 1.97 -> 1.98 ms,
+  // and the node-shifted kernels' LDS would hold one such workgroup per CU)
+  const bool wide16 = (c.storage == MPLBM_FP16S || c.storage == MPLBM_BF16S) &&
+                      c.rounding == MPLBM_ROUND_NEAREST && c.shift != MPLBM_SHIFT_NODE;
+  const int ty_def = wide16 ? 32 : 16;
   if (tz <= 0) tz = c.lz <= 32 ? (int)c.lz : 32;
-  if (ty <= 0) ty = c.ly <= ty_def ? (int)c.ly : ty_def;
+  if (ty <= 0) {
+    ty = c.ly <= ty_def ? (int)c.ly : ty_def;
+    // a workgroup size set without a tile height bounds the default height
+    if (c.block_threads > 0) ty = std::max(1, std::min(ty, c.block_threads / std::min(tz, (int)c.lz)));
+  }
   if (tz > c.lz) tz = (int)c.lz;
   if (ty > c.ly) ty = (int)c.ly;
   if (tz < 1 || ty < 1) return fail(ctx, MPLBM_ERR_ARG, "tile %dx%d too small", ty, tz);
