@@ -666,6 +666,9 @@ def main():
                                       "ms": round(ms, 4), "mlups": round(nf / ms / 1e3, 1)}), flush=True)
         return
     storages = a.storage.split(",")[:1] if a.profile_only else a.storage.split(",")
+    if not a.profile_only and "fp32" not in storages:
+        sys.exit("the bench line is quoted on the FP32 step: --storage must include fp32 "
+                 "(or use --profile-only to time one storage form)")
     res = {}
     for s in storages:
         res[s] = run_gpu(a, shape, fluid, rho, s)
